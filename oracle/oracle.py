@@ -1,0 +1,123 @@
+"""ctypes binding of oracle/liboracle.so (TEST INFRASTRUCTURE ONLY).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module, and only as the checker.  See firmcore_oracle.c for what each
+function restates and how it is pinned.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+_DIR = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_DIR, "liboracle.so")
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            raise RuntimeError("oracle/liboracle.so is not built; run __graft_entry__.build()")
+        _lib = C.CDLL(_LIB_PATH)
+    return _lib
+
+
+def _layer_ptrs(layers):
+    rps = [np.ascontiguousarray(l.row_ptr, dtype=np.int64) for l in layers]
+    cols = [np.ascontiguousarray(l.col, dtype=np.uint32) for l in layers]
+    RP = (C.POINTER(C.c_int64) * len(rps))(*[r.ctypes.data_as(C.POINTER(C.c_int64)) for r in rps])
+    CL = (C.POINTER(C.c_uint32) * len(cols))(*[c.ctypes.data_as(C.POINTER(C.c_uint32)) for c in cols])
+    return RP, CL, (rps, cols)
+
+
+def _i32p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def _i64p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int64))
+
+
+def pargrid(n, layers, cap=None):
+    """ParGrid restatement -> (core_index[L][n], cell_len[L][cap], kmax[L])."""
+    L = len(layers)
+    RP, CL, keep = _layer_ptrs(layers)
+    if cap is None:
+        cap = int(max((l.degrees().max() if l.row_ptr.size > 1 else 0) for l in layers)) + 2
+    ci = np.zeros((L, n), dtype=np.int32)
+    cells = np.zeros((L, cap), dtype=np.int64)
+    kmax = np.zeros(L, dtype=np.int32)
+    rc = lib().orc_pargrid(C.c_int64(n), C.c_int32(L), RP, CL, _i32p(ci), _i64p(cells), C.c_int32(cap), _i32p(kmax))
+    if rc != 0:
+        raise RuntimeError(f"orc_pargrid failed {rc}")
+    return ci, cells, kmax
+
+
+def tree(n, layers, right_spine=False, cap=None):
+    """lpe (right_spine=False) / kpe restatement -> dict."""
+    L = len(layers)
+    RP, CL, keep = _layer_ptrs(layers)
+    if cap is None:
+        cap = int(max((l.degrees().max() if l.row_ptr.size > 1 else 0) for l in layers)) + 2
+    ci = np.zeros((L, n), dtype=np.int32)
+    cells = np.zeros((L, cap), dtype=np.int64)
+    kmax = np.zeros(L, dtype=np.int32)
+    count = C.c_int64(0)
+    sfc = C.c_int64(0)
+    efc = np.zeros(cap, dtype=np.int64)
+    rc = lib().orc_tree(C.c_int64(n), C.c_int32(L), RP, CL, C.c_int32(1 if right_spine else 0), _i32p(ci),
+                        _i64p(cells), C.c_int32(cap), _i32p(kmax), C.byref(count), C.byref(sfc), _i64p(efc),
+                        C.c_int32(cap))
+    if rc != 0:
+        raise RuntimeError(f"orc_tree failed {rc}")
+    return {"core_index": ci, "cell_len": cells, "kmax": kmax, "count": count.value,
+            "sum_num_of_edge_fc": sfc.value, "edge_fc_k1": efc}
+
+
+def coreindex(n, layers):
+    L = len(layers)
+    RP, CL, keep = _layer_ptrs(layers)
+    ci = np.zeros((L, n), dtype=np.int32)
+    rc = lib().orc_coreindex(C.c_int64(n), C.c_int32(L), RP, CL, _i32p(ci))
+    if rc != 0:
+        raise RuntimeError(f"orc_coreindex failed {rc}")
+    return ci
+
+
+def bruteforce(n, layers):
+    L = len(layers)
+    RP, CL, keep = _layer_ptrs(layers)
+    ci = np.zeros((L, n), dtype=np.int32)
+    rc = lib().orc_bruteforce(C.c_int64(n), C.c_int32(L), RP, CL, _i32p(ci))
+    if rc != 0:
+        raise RuntimeError(f"orc_bruteforce failed {rc}")
+    return ci
+
+
+def pow_table(L, beta):
+    """pow(l+1, beta) for l = 0..L-1 via the C library pow, as the reference."""
+    return np.array([math.pow(float(l + 1), float(beta)) for l in range(L)], dtype=np.float64)
+
+
+def wds(n, layers, beta=1.0, cap=None):
+    """WdsList restatement -> dict(max_density float32, res_k, res_lmd, res_len, cell_density, cell_sel)."""
+    L = len(layers)
+    RP, CL, keep = _layer_ptrs(layers)
+    if cap is None:
+        cap = int(max((l.degrees().max() if l.row_ptr.size > 1 else 0) for l in layers)) + 2
+    pw = pow_table(L, beta)
+    md = C.c_float(0)
+    rk, rl, rn = C.c_uint32(0), C.c_uint32(0), C.c_uint32(0)
+    dens = np.zeros((L, cap), dtype=np.float64)
+    sel = np.zeros((L, cap), dtype=np.int32)
+    rc = lib().orc_wds(C.c_int64(n), C.c_int32(L), RP, CL, pw.ctypes.data_as(C.POINTER(C.c_double)), C.byref(md),
+                       C.byref(rk), C.byref(rl), C.byref(rn), dens.ctypes.data_as(C.POINTER(C.c_double)), _i32p(sel),
+                       C.c_int32(cap))
+    if rc != 0:
+        raise RuntimeError(f"orc_wds failed {rc}")
+    return {"max_density": np.float32(md.value), "res_k": rk.value, "res_lmd": rl.value, "res_len": rn.value,
+            "cell_density": dens, "cell_sel": sel}
