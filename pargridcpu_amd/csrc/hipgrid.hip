@@ -1,0 +1,652 @@
+// hipgrid.hip -- C ABI of the HipGrid FirmCore engine (include/hipgrid.h).
+//
+// Device memory layout of one graph (n vertices, L layers, D directed
+// entries over all layers):
+//   off     int64 [n+1]       vertex-major adjacency offsets
+//   adj     uint32 [D]        entry u*L + l (neighbour u in layer l)
+//   deg0    uint32 [n][L]     initial layer degrees
+//   chunks  uint64 [nchunks]  (v, chunk#) list covering adj in kChunk pieces
+// and per instance lmd = 1..L (all allocated once, reused by every call):
+//   deg uint32 [n][L], cnt int32 [n], idx int32 [n] (the core index row),
+//   per-block alive list and removal ring uint32 [G*cap] each (block b of
+//   the fixed grid owns vertices [b*cap, (b+1)*cap)), two deferred chunk
+//   lists uint64 [pcap] (pcap = sum over v of ceil(deg(v) / kDefer)).
+// The graph stays resident between calls; hg_decompose touches no host
+// memory except a small control word per batch of levels.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "hipgrid.h"
+#include "hipgrid_kernels.h"
+
+using namespace hg;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                                  \
+    do {                                                                                               \
+        hipError_t _e = (expr);                                                                        \
+        if (_e != hipSuccess) {                                                                        \
+            return fail(_e == hipErrorOutOfMemory ? HG_ERR_NOMEM : HG_ERR_HIP,                         \
+                        std::string(#expr) + ": " + hipGetErrorString(_e));                           \
+        }                                                                                              \
+    } while (0)
+
+int env_int(const char* name, int dflt) {
+    const char* v = std::getenv(name);
+    if (!v || !*v) return dflt;
+    const int x = std::atoi(v);
+    return x > 0 ? x : dflt;
+}
+
+template <typename T>
+hipError_t dalloc(T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    return hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T));
+}
+
+}  // namespace
+
+struct hg_graph {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int cus = 0;
+    uint64_t wall_khz = 100000;
+    uint32_t n = 0, L = 0, kcap = 0;
+    uint32_t G = 0, cap = 0;  // level-kernel grid and vertices per block
+    uint32_t max_nz = 0;      // most layers in which one vertex has an edge
+    uint64_t D = 0, nchunks = 0, pcap = 0;
+    int64_t device_bytes = 0;
+    // graph
+    int64_t* off = nullptr;
+    uint32_t* adj = nullptr;
+    uint32_t* deg0 = nullptr;
+    uint64_t* chunks = nullptr;
+    // instances
+    uint32_t* deg = nullptr;
+    int32_t* cnt = nullptr;
+    int32_t* idx = nullptr;
+    uint32_t* lists = nullptr;
+    uint32_t* rings = nullptr;
+    uint32_t* blk = nullptr;
+    uint64_t* pl = nullptr;
+    uint64_t* d_ihist = nullptr;  // [L][kcap] core index histogram
+    Inst* d_insts = nullptr;
+    InstState* d_states = nullptr;
+    Control* d_ctl = nullptr;
+    Control* h_ctl = nullptr;  // pinned
+    std::vector<Inst> insts;
+    // densest
+    uint64_t* hist = nullptr;
+    double* d_density = nullptr;
+    int32_t* d_sel = nullptr;
+    int64_t* d_cell_len = nullptr;
+    double* d_pow = nullptr;
+    // results
+    bool decomposed = false;
+    std::vector<int64_t> h_count;    // [L][kcap]: vertices with core index k
+    std::vector<int32_t> h_kmax;     // [L]
+    bool densest_done = false;
+    hg_wds wds{};
+    std::vector<double> h_density;   // [L][kcap]
+    std::vector<int32_t> h_sel;
+    hg_timing timing{};
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+};
+
+namespace {
+
+void release(hg_graph* g) {
+    if (!g) return;
+    (void)hipSetDevice(g->device);
+    void* ptrs[] = {g->off, g->adj, g->deg0, g->chunks, g->deg, g->cnt, g->idx, g->lists, g->rings, g->blk, g->pl,
+                    g->d_ihist, g->d_insts, g->d_states, g->d_ctl, g->hist, g->d_density, g->d_sel, g->d_cell_len,
+                    g->d_pow};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    if (g->h_ctl) (void)hipHostFree(g->h_ctl);
+    for (auto& e : g->ev)
+        if (e) (void)hipEventDestroy(e);
+    delete g;
+}
+
+int create_impl(const hg_layer_csr* layers, int32_t L, int64_t n, int32_t offset_bits, void* stream, hg_graph* g) {
+    if (!layers) return fail(HG_ERR_ARG, "layers is NULL");
+    if (L < 1 || L > 64) return fail(HG_ERR_RANGE, "L must be in [1, 64]");
+    if (n < 1) return fail(HG_ERR_ARG, "n must be >= 1");
+    if (uint64_t(n) * uint64_t(L) >= (1ull << 32)) return fail(HG_ERR_RANGE, "n * L must be < 2^32");
+    if (offset_bits != 32 && offset_bits != 64) return fail(HG_ERR_ARG, "offset_bits must be 32 or 64");
+    for (int l = 0; l < L; ++l)
+        if (!layers[l].row_ptr || (!layers[l].col && n > 0)) return fail(HG_ERR_ARG, "layer pointer is NULL");
+
+    HIP_TRY(hipGetDevice(&g->device));
+    g->stream = static_cast<hipStream_t>(stream);
+    HIP_TRY(hipDeviceGetAttribute(&g->cus, hipDeviceAttributeMultiprocessorCount, g->device));
+    int wall = 0;
+    if (hipDeviceGetAttribute(&wall, hipDeviceAttributeWallClockRate, g->device) == hipSuccess && wall > 0)
+        g->wall_khz = uint64_t(wall);
+    g->n = uint32_t(n);
+    g->L = uint32_t(L);
+
+    // Host pass over the offsets: per-layer int64 offsets, per-vertex totals,
+    // merged offsets, chunk offsets, max layer degree.
+    std::vector<std::vector<int64_t>> rp(static_cast<size_t>(L));
+    uint64_t maxdeg = 0;
+    for (int l = 0; l < L; ++l) {
+        rp[size_t(l)].resize(size_t(n) + 1);
+        int64_t* dst = rp[size_t(l)].data();
+        if (offset_bits == 64) {
+            std::memcpy(dst, layers[l].row_ptr, sizeof(int64_t) * (size_t(n) + 1));
+        } else {
+            const int32_t* src = static_cast<const int32_t*>(layers[l].row_ptr);
+            for (int64_t v = 0; v <= n; ++v) dst[v] = src[v];
+        }
+        if (dst[0] != 0) return fail(HG_ERR_ARG, "row_ptr[0] must be 0");
+        for (int64_t v = 0; v < n; ++v) {
+            if (dst[v + 1] < dst[v]) return fail(HG_ERR_ARG, "row_ptr must be non-decreasing");
+            maxdeg = std::max<uint64_t>(maxdeg, uint64_t(dst[v + 1] - dst[v]));
+        }
+    }
+    std::vector<int64_t> off(size_t(n) + 1);
+    std::vector<uint64_t> coff(size_t(n) + 1);
+    off[0] = 0;
+    coff[0] = 0;
+    uint64_t pcap = 0;
+    uint32_t max_nz = 0;
+    for (int64_t v = 0; v < n; ++v) {
+        int64_t t = 0;
+        uint32_t nz = 0;
+        for (int l = 0; l < L; ++l) {
+            const int64_t d = rp[size_t(l)][size_t(v) + 1] - rp[size_t(l)][size_t(v)];
+            t += d;
+            nz += d > 0;
+        }
+        max_nz = std::max(max_nz, nz);
+        off[size_t(v) + 1] = off[size_t(v)] + t;
+        coff[size_t(v) + 1] = coff[size_t(v)] + uint64_t((t + kChunk - 1) / kChunk);
+        pcap += uint64_t((t + kDefer - 1) / kDefer);
+    }
+    g->D = uint64_t(off[size_t(n)]);
+    if (g->D >= (1ull << 40)) return fail(HG_ERR_RANGE, "total directed entries must be < 2^40");
+    g->nchunks = coff[size_t(n)];
+    g->pcap = pcap + 1;
+    g->max_nz = max_nz;
+    g->kcap = uint32_t(maxdeg + 2);
+    // Level-kernel grid: HG_BLOCKS overrides 2 blocks per CU.
+    const uint32_t want_g = uint32_t(env_int("HG_BLOCKS", 2 * g->cus));
+    g->G = std::max(1u, std::min<uint32_t>(want_g, uint32_t(n)));
+    g->cap = uint32_t((uint64_t(n) + g->G - 1) / g->G);
+
+    const size_t nL = size_t(n) * size_t(L);
+    int64_t bytes = 0;
+    auto acct = [&](size_t b) { bytes += int64_t(b); };
+    HIP_TRY(dalloc(&g->off, size_t(n) + 1)); acct(8 * (size_t(n) + 1));
+    HIP_TRY(dalloc(&g->adj, g->D)); acct(4 * g->D);
+    HIP_TRY(dalloc(&g->deg0, nL)); acct(4 * nL);
+    HIP_TRY(dalloc(&g->chunks, g->nchunks)); acct(8 * g->nchunks);
+    HIP_TRY(dalloc(&g->deg, nL * L)); acct(4 * nL * L);
+    HIP_TRY(dalloc(&g->cnt, nL)); acct(4 * nL);
+    HIP_TRY(dalloc(&g->idx, nL)); acct(4 * nL);
+    const size_t gcap = size_t(g->G) * g->cap;
+    HIP_TRY(dalloc(&g->lists, gcap * L)); acct(4 * gcap * L);
+    HIP_TRY(dalloc(&g->rings, gcap * L)); acct(4 * gcap * L);
+    HIP_TRY(dalloc(&g->blk, size_t(2) * g->G * L)); acct(8 * size_t(g->G) * L);
+    HIP_TRY(dalloc(&g->pl, size_t(2) * g->pcap * L)); acct(16 * size_t(g->pcap) * L);
+    HIP_TRY(dalloc(&g->d_ihist, size_t(g->kcap) * L)); acct(8 * size_t(g->kcap) * L);
+    HIP_TRY(dalloc(&g->d_insts, size_t(L)));
+    HIP_TRY(dalloc(&g->d_states, size_t(L)));
+    HIP_TRY(dalloc(&g->d_ctl, 1));
+    HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&g->h_ctl), sizeof(Control), hipHostMallocDefault));
+    HIP_TRY(dalloc(&g->d_density, size_t(g->kcap)));
+    HIP_TRY(dalloc(&g->d_sel, size_t(g->kcap)));
+    HIP_TRY(dalloc(&g->d_cell_len, size_t(g->kcap)));
+    HIP_TRY(dalloc(&g->d_pow, size_t(L)));
+    for (auto& e : g->ev) HIP_TRY(hipEventCreate(&e));
+    g->device_bytes = bytes;
+
+    // Upload per-layer CSR into temporaries, build deg0 / adj / chunks.
+    std::vector<int64_t*> d_rp(static_cast<size_t>(L), nullptr);
+    std::vector<uint32_t*> d_col(static_cast<size_t>(L), nullptr);
+    int64_t** d_rp_arr = nullptr;
+    uint32_t** d_col_arr = nullptr;
+    uint64_t* d_coff = nullptr;
+    auto cleanup = [&]() {
+        for (auto p : d_rp) if (p) (void)hipFree(p);
+        for (auto p : d_col) if (p) (void)hipFree(p);
+        if (d_rp_arr) (void)hipFree(d_rp_arr);
+        if (d_col_arr) (void)hipFree(d_col_arr);
+        if (d_coff) (void)hipFree(d_coff);
+    };
+    hipError_t e = hipSuccess;
+    for (int l = 0; l < L && e == hipSuccess; ++l) {
+        const size_t m = size_t(rp[size_t(l)][size_t(n)]);
+        e = dalloc(&d_rp[size_t(l)], size_t(n) + 1);
+        if (e == hipSuccess) e = dalloc(&d_col[size_t(l)], m);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(d_rp[size_t(l)], rp[size_t(l)].data(), sizeof(int64_t) * (size_t(n) + 1),
+                               hipMemcpyHostToDevice, g->stream);
+        if (e == hipSuccess && m)
+            e = hipMemcpyAsync(d_col[size_t(l)], layers[l].col, sizeof(uint32_t) * m, hipMemcpyHostToDevice,
+                               g->stream);
+    }
+    if (e == hipSuccess) e = dalloc(&d_rp_arr, size_t(L));
+    if (e == hipSuccess) e = dalloc(&d_col_arr, size_t(L));
+    if (e == hipSuccess) e = dalloc(&d_coff, size_t(n) + 1);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d_rp_arr, d_rp.data(), sizeof(int64_t*) * L, hipMemcpyHostToDevice, g->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d_col_arr, d_col.data(), sizeof(uint32_t*) * L, hipMemcpyHostToDevice, g->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(g->off, off.data(), sizeof(int64_t) * (size_t(n) + 1), hipMemcpyHostToDevice, g->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d_coff, coff.data(), sizeof(uint64_t) * (size_t(n) + 1), hipMemcpyHostToDevice, g->stream);
+    if (e == hipSuccess) {
+        const uint32_t grid = uint32_t(std::min<uint64_t>((nL + kBlock - 1) / kBlock, uint64_t(g->cus) * 16));
+        hipLaunchKernelGGL(k_build_deg0, dim3(grid), dim3(kBlock), 0, g->stream, d_rp_arr, g->n, g->L, g->deg0);
+        const uint32_t grid2 = uint32_t(std::min<uint64_t>((uint64_t(n) * kWave + kBlock - 1) / kBlock,
+                                                           uint64_t(g->cus) * 16));
+        hipLaunchKernelGGL(k_build_adj, dim3(grid2), dim3(kBlock), 0, g->stream, d_rp_arr, d_col_arr, g->deg0,
+                           g->off, g->n, g->L, g->adj);
+        const uint32_t grid3 = uint32_t(std::min<uint64_t>((uint64_t(n) + kBlock - 1) / kBlock, uint64_t(g->cus) * 16));
+        hipLaunchKernelGGL(k_build_chunks, dim3(grid3), dim3(kBlock), 0, g->stream, g->off, d_coff, g->n, g->chunks);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
+    cleanup();
+    if (e != hipSuccess)
+        return fail(e == hipErrorOutOfMemory ? HG_ERR_NOMEM : HG_ERR_HIP,
+                    std::string("graph upload: ") + hipGetErrorString(e));
+
+    g->insts.resize(size_t(L));
+    for (uint32_t i = 0; i < g->L; ++i) {
+        Inst& in = g->insts[i];
+        in.off = g->off;
+        in.adj = g->adj;
+        in.deg = g->deg + size_t(i) * nL;
+        in.cnt = g->cnt + size_t(i) * n;
+        in.idx = g->idx + size_t(i) * n;
+        in.list = g->lists + size_t(i) * gcap;
+        in.ring = g->rings + size_t(i) * gcap;
+        in.blk = g->blk + size_t(2) * g->G * i;
+        in.pl0 = g->pl + size_t(2 * i) * g->pcap;
+        in.pl1 = g->pl + size_t(2 * i + 1) * g->pcap;
+        in.plcap = g->pcap;
+        in.n = g->n;
+        in.L = g->L;
+        in.lmd = i + 1;
+        in.cap = g->cap;
+        in.lmagic = g->L == 1 ? 0xFFFFFFFFu : uint32_t((1ull << 32) / g->L);  // div_L needs magic < 2^32
+        in.flags = env_int("HG_AGG", 1) == 2 ? 0u : kFlagAggregate;  // HG_AGG=2 disables combining
+    }
+    return HG_OK;
+}
+
+int sync_ctl(hg_graph* g) {
+    HIP_TRY(hipMemcpyAsync(g->h_ctl, g->d_ctl, sizeof(Control), hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    return HG_OK;
+}
+
+const char* err_name(int32_t e) {
+    switch (e) {
+        case kErrDefer: return "deferred chunk list overflow";
+        default: return "unknown device error";
+    }
+}
+
+// Launches step s: k_advance, then the scan (level-0 init at s = 0), then
+// the cascade.  Kernels of instances that are done return at once.
+void launch_step(hg_graph* g, uint32_t s) {
+    const uint32_t L = g->L;
+    hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, g->stream, g->d_states, g->d_ctl, L, s);
+    if (s == 0)
+        hipLaunchKernelGGL(k_init, dim3(g->G), dim3(kBlock), 0, g->stream, g->d_insts, g->d_states, g->d_ctl, L,
+                           g->deg0, g->n, g->L, g->cap);
+    else
+        hipLaunchKernelGGL(k_scan, dim3(g->G), dim3(kBlock), 0, g->stream, g->d_insts, g->d_states, g->d_ctl, L, s);
+    hipLaunchKernelGGL(k_cascade, dim3(g->G), dim3(kBlock), 0, g->stream, g->d_insts, g->d_states, g->d_ctl, L, s);
+}
+
+int decompose_impl(hg_graph* g) {
+    HIP_TRY(hipSetDevice(g->device));
+    const uint32_t L = g->L;
+    g->decomposed = false;
+    g->densest_done = false;
+    std::vector<InstState> st(L);
+    std::memset(st.data(), 0, sizeof(InstState) * L);
+    for (uint32_t i = 0; i < L; ++i) {
+        st[i].alive[0] = g->n;
+        if (i + 1 > g->max_nz) {
+            // No vertex has edges in lmd layers: S(1, lmd) is empty, every index is 0.
+            st[i].phase = kPhaseDone;
+            st[i].alive[0] = 0;
+            HIP_TRY(hipMemsetAsync(g->insts[i].idx, 0, sizeof(int32_t) * g->n, g->stream));
+        }
+    }
+    Control ctl;
+    std::memset(&ctl, 0, sizeof ctl);
+    ctl.active = int32_t(L);
+    HIP_TRY(hipMemcpyAsync(g->d_insts, g->insts.data(), sizeof(Inst) * L, hipMemcpyHostToDevice, g->stream));
+    HIP_TRY(hipMemcpyAsync(g->d_states, st.data(), sizeof(InstState) * L, hipMemcpyHostToDevice, g->stream));
+    HIP_TRY(hipMemcpyAsync(g->d_ctl, &ctl, sizeof(Control), hipMemcpyHostToDevice, g->stream));
+
+    HIP_TRY(hipEventRecord(g->ev[0], g->stream));
+    uint32_t s = 0;
+    launch_step(g, s++);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(g->ev[1], g->stream));
+    // Steps run in batches; the host only reads the active count between
+    // batches (a finished instance makes its kernels return at once).
+    static const int batch_max = env_int("HG_BATCH", 32);
+    uint32_t batch = 4;
+    const uint64_t step_bound = 4ull * (uint64_t(g->kcap) + 8) + 2ull * g->n + 64;
+    for (;;) {
+        for (uint32_t b = 0; b < batch; ++b) launch_step(g, s++);
+        HIP_TRY(hipGetLastError());
+        int rc = sync_ctl(g);
+        if (rc) return rc;
+        if (g->h_ctl->error) return fail(HG_ERR_DEVICE, err_name(g->h_ctl->error));
+        if (g->h_ctl->active == 0) break;
+        if (s > step_bound) return fail(HG_ERR_DEVICE, "step loop did not terminate");
+        batch = std::min<uint32_t>(batch * 2, uint32_t(batch_max));
+    }
+    // Cell lengths and kmax from the histogram of core indices.
+    HIP_TRY(hipMemsetAsync(g->d_ihist, 0, sizeof(uint64_t) * size_t(g->kcap) * L, g->stream));
+    const uint32_t hx = uint32_t(std::min<uint64_t>((uint64_t(g->n) + kBlock - 1) / kBlock, uint64_t(g->cus) * 2));
+    hipLaunchKernelGGL(k_idx_hist, dim3(std::max(hx, 1u), L), dim3(kBlock), 0, g->stream, g->idx, g->n, L, g->d_ihist,
+                       g->kcap);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(g->ev[2], g->stream));
+    HIP_TRY(hipEventSynchronize(g->ev[2]));
+    float ms_all = 0, ms_init = 0;
+    HIP_TRY(hipEventElapsedTime(&ms_all, g->ev[0], g->ev[2]));
+    HIP_TRY(hipEventElapsedTime(&ms_init, g->ev[0], g->ev[1]));
+    g->timing.decompose_ms = ms_all;
+    g->timing.init_ms = ms_init;
+    g->timing.scan_ms = -1;
+    g->timing.cascade_ms = -1;
+    g->timing.levels = int64_t(s);
+    g->timing.launches = int64_t(s) * 3 + 1;
+
+    g->h_count.assign(size_t(g->kcap) * L, 0);
+    HIP_TRY(hipMemcpy(g->h_count.data(), g->d_ihist, sizeof(int64_t) * g->h_count.size(), hipMemcpyDeviceToHost));
+    g->h_kmax.assign(L, 0);
+    int64_t total = 0;
+    for (uint32_t i = 0; i < L; ++i) {
+        for (uint32_t k = 0; k < g->kcap; ++k) {
+            const int64_t c = g->h_count[size_t(i) * g->kcap + k];
+            total += c;
+            if (c > 0) g->h_kmax[i] = int32_t(k);
+        }
+    }
+    if (total != int64_t(g->n) * L)
+        return fail(HG_ERR_DEVICE, "core index histogram covers " + std::to_string(total) + " of " +
+                                       std::to_string(int64_t(g->n) * L) + " entries");
+    g->decomposed = true;
+    return HG_OK;
+}
+
+std::vector<int64_t> cell_lengths(const hg_graph* g, uint32_t i) {
+    // |S(k)| = #{v : index >= k}, k = 0..kmax
+    const int32_t km = g->h_kmax[i];
+    std::vector<int64_t> len(size_t(km) + 1);
+    int64_t acc = 0;
+    for (int32_t k = km; k >= 0; --k) {
+        acc += g->h_count[size_t(i) * g->kcap + size_t(k)];
+        len[size_t(k)] = acc;
+    }
+    return len;
+}
+
+int densest_impl(hg_graph* g, double beta, hg_wds* out) {
+    HIP_TRY(hipSetDevice(g->device));
+    const uint32_t L = g->L;
+    std::vector<double> pw(L);
+    for (uint32_t l = 0; l < L; ++l) pw[l] = std::pow(double(l + 1), beta);
+    HIP_TRY(hipMemcpyAsync(g->d_pow, pw.data(), sizeof(double) * L, hipMemcpyHostToDevice, g->stream));
+    int32_t kmax_all = 0;
+    for (uint32_t i = 0; i < L; ++i) kmax_all = std::max(kmax_all, g->h_kmax[i]);
+    const size_t kh = size_t(kmax_all) + 1;
+    if (!g->hist) {
+        hipError_t e = dalloc(&g->hist, size_t(L) * g->kcap);
+        if (e != hipSuccess) return fail(HG_ERR_NOMEM, "densest histogram allocation failed");
+    }
+    g->h_density.assign(size_t(g->kcap) * L, 0.0);
+    g->h_sel.assign(size_t(g->kcap) * L, 0);
+    HIP_TRY(hipEventRecord(g->ev[0], g->stream));
+    const uint32_t grid = uint32_t(std::min<uint64_t>((g->nchunks * kChunk / 64 + kHistBlock - 1) / kHistBlock,
+                                                       uint64_t(g->cus) * 8));
+    for (uint32_t i = 0; i < L; ++i) {
+        const int32_t km = g->h_kmax[i];
+        if (km < 1) continue;
+        const uint32_t kc = uint32_t(km) + 1;
+        HIP_TRY(hipMemsetAsync(g->hist, 0, sizeof(uint64_t) * size_t(L) * kc, g->stream));
+        hipLaunchKernelGGL(k_edge_hist, dim3(std::max(grid, 1u)), dim3(kHistBlock), 0, g->stream, g->insts[i], g->chunks,
+                           g->nchunks, g->hist, kc);
+        std::vector<int64_t> len = cell_lengths(g, i);
+        HIP_TRY(hipMemcpyAsync(g->d_cell_len, len.data(), sizeof(int64_t) * len.size(), hipMemcpyHostToDevice,
+                               g->stream));
+        const uint32_t cells = uint32_t(km);
+        hipLaunchKernelGGL(k_cell_score, dim3((cells + 63) / 64), dim3(64), 0, g->stream, g->hist, kc, L, uint32_t(km),
+                           g->d_cell_len, g->d_pow, g->d_density, g->d_sel);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(g->h_density.data() + size_t(i) * g->kcap, g->d_density, sizeof(double) * (size_t(km) + 1),
+                               hipMemcpyDeviceToHost, g->stream));
+        HIP_TRY(hipMemcpyAsync(g->h_sel.data() + size_t(i) * g->kcap, g->d_sel, sizeof(int32_t) * (size_t(km) + 1),
+                               hipMemcpyDeviceToHost, g->stream));
+        HIP_TRY(hipStreamSynchronize(g->stream));  // host buffers are reused per instance
+        g->h_density[size_t(i) * g->kcap] = 0.0;
+        g->h_sel[size_t(i) * g->kcap] = 0;
+    }
+    HIP_TRY(hipEventRecord(g->ev[1], g->stream));
+    HIP_TRY(hipEventSynchronize(g->ev[1]));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, g->ev[0], g->ev[1]));
+    g->timing.densest_ms = ms;
+    (void)kh;
+    // Argmax exactly as FCTree::WeightDenestSubgraph with main.cpp's float
+    // running max: pre-order = lambda-major, k ascending; update on >=.
+    hg_wds r{};
+    float maxd = 0.0f;
+    for (uint32_t i = 0; i < L; ++i) {
+        const std::vector<int64_t> len = cell_lengths(g, i);
+        for (int32_t k = 1; k <= g->h_kmax[i]; ++k) {
+            const double d = g->h_density[size_t(i) * g->kcap + size_t(k)];
+            if (d >= double(maxd)) {
+                maxd = float(d);
+                r.res_k = uint32_t(k);
+                r.res_lmd = i + 1;
+                r.res_len = uint32_t(len[size_t(k)]);
+                r.num_select_layer = g->h_sel[size_t(i) * g->kcap + size_t(k)];
+            }
+        }
+    }
+    r.max_density = maxd;
+    g->wds = r;
+    g->densest_done = true;
+    if (out) *out = r;
+    return HG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* hg_last_error(void) { return g_err.c_str(); }
+
+int hg_graph_create(const hg_layer_csr* layers, int32_t L, int64_t n, int32_t offset_bits, void* stream,
+                    hg_graph** out) {
+    if (!out) return fail(HG_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    hg_graph* g = new (std::nothrow) hg_graph;
+    if (!g) return fail(HG_ERR_NOMEM, "host allocation failed");
+    int rc;
+    try {
+        rc = create_impl(layers, L, n, offset_bits, stream, g);
+    } catch (const std::bad_alloc&) {
+        rc = fail(HG_ERR_NOMEM, "host allocation failed in hg_graph_create");
+    }
+    if (rc != HG_OK) {
+        release(g);
+        return rc;
+    }
+    *out = g;
+    return HG_OK;
+}
+
+int hg_decompose(hg_graph* g) {
+    if (!g) return fail(HG_ERR_ARG, "graph is NULL");
+    try {
+        return decompose_impl(g);
+    } catch (const std::bad_alloc&) {
+        return fail(HG_ERR_NOMEM, "host allocation failed in hg_decompose");
+    }
+}
+
+int hg_core_index(hg_graph* g, int32_t* core_index) {
+    if (!g || !core_index) return fail(HG_ERR_ARG, "NULL argument");
+    if (!g->decomposed) return fail(HG_ERR_STATE, "hg_decompose has not completed");
+    HIP_TRY(hipSetDevice(g->device));
+    HIP_TRY(hipMemcpyAsync(core_index, g->idx, sizeof(int32_t) * size_t(g->n) * g->L, hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    return HG_OK;
+}
+
+int hg_core_index_device(hg_graph* g, const int32_t** dev_ptr) {
+    if (!g || !dev_ptr) return fail(HG_ERR_ARG, "NULL argument");
+    if (!g->decomposed) return fail(HG_ERR_STATE, "hg_decompose has not completed");
+    *dev_ptr = g->idx;
+    return HG_OK;
+}
+
+int hg_cells(hg_graph* g, int64_t* cell_len, int32_t cap, int32_t* kmax) {
+    if (!g) return fail(HG_ERR_ARG, "graph is NULL");
+    if (cell_len && cap < 1) return fail(HG_ERR_ARG, "cap must be >= 1");
+    if (!g->decomposed) return fail(HG_ERR_STATE, "hg_decompose has not completed");
+    for (uint32_t i = 0; i < g->L; ++i) {
+        if (kmax) kmax[i] = g->h_kmax[i];
+        if (cell_len) {
+            const std::vector<int64_t> len = cell_lengths(g, i);
+            for (int32_t k = 0; k < cap; ++k)
+                cell_len[size_t(i) * cap + size_t(k)] = k < int32_t(len.size()) ? len[size_t(k)] : 0;
+        }
+    }
+    return HG_OK;
+}
+
+int hg_shells(hg_graph* g, int32_t kcap, int64_t* shell_off, uint32_t* shell_vtx) {
+    if (!g || !shell_off || !shell_vtx) return fail(HG_ERR_ARG, "NULL argument");
+    if (!g->decomposed) return fail(HG_ERR_STATE, "hg_decompose has not completed");
+    for (uint32_t i = 0; i < g->L; ++i)
+        if (g->h_kmax[i] >= kcap) return fail(HG_ERR_ARG, "kcap must exceed every kmax");
+    try {
+        std::vector<int32_t> ci(size_t(g->n) * g->L);
+        int rc = hg_core_index(g, ci.data());
+        if (rc) return rc;
+        const size_t stride = size_t(kcap) + 1;
+        std::vector<int64_t> cnt(stride * g->L + 1, 0);
+        for (uint32_t i = 0; i < g->L; ++i)
+            for (uint32_t v = 0; v < g->n; ++v) cnt[i * stride + size_t(ci[size_t(i) * g->n + v]) + 1]++;
+        shell_off[0] = 0;
+        for (size_t t = 0; t < stride * g->L; ++t) shell_off[t + 1] = shell_off[t] + cnt[t + 1];
+        std::vector<int64_t> pos(shell_off, shell_off + stride * g->L);
+        for (uint32_t i = 0; i < g->L; ++i)
+            for (uint32_t v = 0; v < g->n; ++v) shell_vtx[pos[i * stride + size_t(ci[size_t(i) * g->n + v])]++] = v;
+    } catch (const std::bad_alloc&) {
+        return fail(HG_ERR_NOMEM, "host allocation failed in hg_shells");
+    }
+    return HG_OK;
+}
+
+int hg_densest(hg_graph* g, double beta, hg_wds* out) {
+    if (!g) return fail(HG_ERR_ARG, "graph is NULL");
+    if (!g->decomposed) return fail(HG_ERR_STATE, "hg_decompose has not completed");
+    if (!std::isfinite(beta)) return fail(HG_ERR_ARG, "beta must be finite");
+    try {
+        return densest_impl(g, beta, out);
+    } catch (const std::bad_alloc&) {
+        return fail(HG_ERR_NOMEM, "host allocation failed in hg_densest");
+    }
+}
+
+int hg_cell_density(hg_graph* g, double* density, int32_t* num_select_layer, int32_t cap) {
+    if (!g) return fail(HG_ERR_ARG, "graph is NULL");
+    if (!g->densest_done) return fail(HG_ERR_STATE, "hg_densest has not completed");
+    if (cap < 1) return fail(HG_ERR_ARG, "cap must be >= 1");
+    for (uint32_t i = 0; i < g->L; ++i)
+        for (int32_t k = 0; k < cap; ++k) {
+            const bool in = k <= g->h_kmax[i] && uint32_t(k) < g->kcap;
+            if (density) density[size_t(i) * cap + size_t(k)] = in ? g->h_density[size_t(i) * g->kcap + size_t(k)] : 0.0;
+            if (num_select_layer)
+                num_select_layer[size_t(i) * cap + size_t(k)] = in ? g->h_sel[size_t(i) * g->kcap + size_t(k)] : 0;
+        }
+    return HG_OK;
+}
+
+int hg_densest_vertices(hg_graph* g, uint32_t* out, int64_t cap, int64_t* count) {
+    if (!g || !count) return fail(HG_ERR_ARG, "NULL argument");
+    if (!g->densest_done) return fail(HG_ERR_STATE, "hg_densest has not completed");
+    *count = 0;
+    if (g->wds.res_lmd == 0) return HG_OK;
+    try {
+        std::vector<int32_t> row(g->n);
+        HIP_TRY(hipSetDevice(g->device));
+        HIP_TRY(hipMemcpyAsync(row.data(), g->idx + size_t(g->wds.res_lmd - 1) * g->n, sizeof(int32_t) * g->n,
+                               hipMemcpyDeviceToHost, g->stream));
+        HIP_TRY(hipStreamSynchronize(g->stream));
+        int64_t c = 0;
+        for (uint32_t v = 0; v < g->n; ++v) {
+            if (row[v] >= int32_t(g->wds.res_k)) {
+                if (out && c < cap) out[c] = v;
+                ++c;
+            }
+        }
+        *count = c;
+        if (out && c > cap) return fail(HG_ERR_ARG, "output capacity too small");
+    } catch (const std::bad_alloc&) {
+        return fail(HG_ERR_NOMEM, "host allocation failed in hg_densest_vertices");
+    }
+    return HG_OK;
+}
+
+int hg_last_timing(hg_graph* g, hg_timing* out) {
+    if (!g || !out) return fail(HG_ERR_ARG, "NULL argument");
+    *out = g->timing;
+    return HG_OK;
+}
+
+int hg_graph_info(hg_graph* g, int64_t* n, int32_t* L, int64_t* directed_entries, int64_t* device_bytes) {
+    if (!g) return fail(HG_ERR_ARG, "graph is NULL");
+    if (n) *n = g->n;
+    if (L) *L = int32_t(g->L);
+    if (directed_entries) *directed_entries = int64_t(g->D);
+    if (device_bytes) *device_bytes = g->device_bytes;
+    return HG_OK;
+}
+
+int hg_graph_destroy(hg_graph* g) {
+    if (!g) return fail(HG_ERR_ARG, "graph is NULL");
+    (void)hipSetDevice(g->device);
+    if (g->stream) (void)hipStreamSynchronize(g->stream);
+    else (void)hipDeviceSynchronize();
+    release(g);
+    return HG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,736 @@
+// hipgrid_kernels.h -- CDNA4 (gfx950) kernels of the HipGrid FirmCore engine.
+//
+// Work model.  A decomposition runs one peel instance per lmd = 1..L over a
+// graph resident in HBM.  Per instance and vertex: the L layer degrees in
+// the residual graph (deg, AoS [n][L]: one vertex's row is one or two
+// 32-byte sectors), cnt = number of layers with degree >= the current
+// threshold, and idx = core index (-1 while alive).  Level k removes every
+// alive vertex whose index is k, turning S(k, lmd) into S(k+1, lmd).  This
+// is FCSyncLeft::PeelSync's two-phase rule (CoreParallel/FCSyncLeft.cpp:
+// 65-108):
+//   scan     a vertex with fewer than lmd layers of degree >= k+1 is removed,
+//            the others get cnt;
+//   cascade  each entry of a removed vertex decrements deg[u][l] with an
+//            atomic; the unique decrement taking it from k+1 to k decrements
+//            cnt[u]; the unique decrement taking cnt[u] from lmd to lmd-1
+//            removes u (CAS on idx).
+//
+// Scheduling, designed around the cost of same-address device atomics
+// (they serialise, ~11 ns each on MI355X, while scattered atomics reach
+// ~26 G/s):
+//   * The grid is G fixed blocks.  Block b owns the vertices v = b mod G for
+//     the scan (strided, so the hubs of a power-law graph, which cluster at
+//     low ids in the generator, spread over all blocks) and keeps their
+//     alive list compacted in place in its own list slice: no global
+//     counters.
+//   * Removals go to the discovering block's own FIFO ring (its ring slice,
+//     slots reserved with LDS atomics) and are expanded by that block.  The
+//     only global queue is the deferred list: adjacency chunks of heavy
+//     vertices (> kHeavy entries) and of ring overflow.  Deferred chunks are
+//     processed in the next launch, statically strided over all blocks, so a
+//     hub is spread over the whole GPU and nothing ever spins.
+//   * One step = k_advance (1 block: per-instance state machine) + A (scan,
+//     or level-0 init) + B (cascade).  A step either starts a new level or
+//     continues the current one while deferred chunks remain.
+//
+// Adjacency is vertex-major: entries of all layers of v are contiguous at
+// adj[off[v] .. off[v+1]); an entry is u*L + l, which is also the index of
+// deg[u][l].
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace hg {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 512;  // threads per block of the level kernels (8 wavefronts)
+constexpr int kWavesPerBlock = kBlock / kWave;
+constexpr int kChunk = 64;         // static chunk list granularity (densest pass)
+constexpr uint32_t kHeavy = 4096;  // vertices with more entries are deferred
+constexpr uint32_t kDefer = 4096;  // entries per deferred chunk
+
+enum Phase : int32_t { kPhaseScan = 0, kPhaseCont = 1, kPhaseDone = 2 };
+
+// Per-instance constant data (one lmd of one graph).
+struct Inst {
+    const int64_t* off;   // [n+1] merged adjacency offsets
+    const uint32_t* adj;  // [D] entries u*L + l
+    uint32_t* deg;        // [n*L] residual layer degrees
+    int32_t* cnt;         // [n]
+    int32_t* idx;         // [n] core index, -1 while alive
+    uint32_t* list;       // [G*cap] per-block alive lists
+    uint32_t* ring;       // [G*cap] per-block removal rings
+    uint32_t* blk;        // [G*2] per block: alive list length, ring count from the scan
+    uint64_t* pl0;        // deferred chunk lists, by step parity
+    uint64_t* pl1;
+    uint64_t plcap;
+    uint32_t n, L, lmd, cap;
+    uint32_t lmagic;  // min(floor(2^32 / L), 2^32 - 1)
+    uint32_t flags;   // kFlagAggregate: combine decrements in LDS
+};
+constexpr uint32_t kFlagAggregate = 1;
+
+// Per-instance state; the atomically updated words sit on lines of their own.
+struct alignas(128) InstState {
+    int32_t phase;
+    int32_t k;
+    int32_t pad0[30];
+    unsigned long long pcount[2][16];  // deferred chunk counts (one line each)
+    unsigned long long alive[16];      // vertices not yet removed (decremented mod 2^64)
+};
+
+struct alignas(128) Control {
+    int32_t error;   // nonzero: device-side check failed
+    int32_t active;  // instances not done after the last k_advance
+    int32_t pad[30];
+};
+
+enum Err : int32_t { kErrNone = 0, kErrDefer = 2 };
+
+// ----------------------------------------------------------------- helpers
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// e / L for e < 2^32 with magic = min(floor(2^32 / L), 2^32 - 1): the
+// multiply-high estimate is at most one below the quotient.
+__device__ __forceinline__ uint32_t div_L(uint32_t e, uint32_t L, uint32_t magic) {
+    uint32_t q = __umulhi(e, magic);
+    if (e - q * L >= L) ++q;
+    return q;
+}
+
+// Exclusive prefix sum of x over the wavefront; *total gets the sum.
+__device__ __forceinline__ uint32_t wave_excl_scan(uint32_t x, uint32_t* total) {
+    const uint32_t lane = lane_id();
+    uint32_t s = x;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const uint32_t y = __shfl_up(s, d, kWave);
+        if (lane >= uint32_t(d)) s += y;
+    }
+    *total = __shfl(s, kWave - 1, kWave);
+    return s - x;
+}
+
+// Exclusive prefix sum over the block; s_tmp needs kWavesPerBlock + 1 words.
+// Must be called by every thread of the block.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t x, uint32_t* total, uint32_t* s_tmp) {
+    const uint32_t lane = lane_id(), wid = threadIdx.x / kWave;
+    uint32_t wt;
+    const uint32_t wex = wave_excl_scan(x, &wt);
+    if (lane == 0) s_tmp[wid] = wt;
+    __syncthreads();
+    if (wid == 0) {
+        const uint32_t v = lane < uint32_t(kWavesPerBlock) ? s_tmp[lane] : 0u;
+        uint32_t t;
+        const uint32_t e = wave_excl_scan(v, &t);
+        if (lane < uint32_t(kWavesPerBlock)) s_tmp[lane] = e;
+        if (lane == 0) s_tmp[kWavesPerBlock] = t;
+    }
+    __syncthreads();
+    *total = s_tmp[kWavesPerBlock];
+    const uint32_t r = wex + s_tmp[wid];
+    __syncthreads();
+    return r;
+}
+
+__device__ __forceinline__ uint64_t make_item(uint64_t start, uint32_t len) { return start | (uint64_t(len) << 40); }
+__device__ __forceinline__ uint64_t item_start(uint64_t it) { return it & ((1ull << 40) - 1); }
+__device__ __forceinline__ uint32_t item_len(uint64_t it) { return uint32_t(it >> 40); }
+
+__device__ __forceinline__ uint64_t* defer_list(const Inst& in, uint32_t parity) { return parity ? in.pl1 : in.pl0; }
+
+// Wave-aggregated append of the adjacency of the lanes' vertices (defer ==
+// true) to a deferred list, in chunks of kDefer entries.  All lanes call.
+__device__ __forceinline__ void defer_push(const Inst& in, uint64_t* list, unsigned long long* count, Control* ctl,
+                                           bool defer, uint32_t v) {
+    uint64_t beg = 0, dt = 0;
+    uint32_t nch = 0;
+    if (defer) {
+        beg = uint64_t(in.off[v]);
+        dt = uint64_t(in.off[v + 1]) - beg;
+        nch = uint32_t((dt + kDefer - 1) / kDefer);
+    }
+    if (__ballot(nch != 0) == 0) return;
+    uint32_t total;
+    const uint32_t ex = wave_excl_scan(nch, &total);
+    unsigned long long base = 0;
+    if (lane_id() == 0) base = atomicAdd(count, (unsigned long long)total);
+    base = __shfl(base, 0, kWave);
+    if (base + total > in.plcap) {  // cannot happen: plcap bounds all deferrals
+        if (lane_id() == 0) atomicCAS(&ctl->error, 0, int32_t(kErrDefer));
+        return;
+    }
+    for (uint32_t c = 0; c < nch; ++c) {
+        const uint64_t rem = dt - uint64_t(c) * kDefer;
+        list[base + ex + c] = make_item(beg + uint64_t(c) * kDefer, rem < kDefer ? uint32_t(rem) : kDefer);
+    }
+}
+
+// ----------------------------------------------------------- graph set-up
+// deg0[v][l] = deg_l(v) from per-layer offsets.
+__global__ void k_build_deg0(const int64_t* const* row_ptr, uint32_t n, uint32_t L, uint32_t* deg0) {
+    const uint64_t total = uint64_t(n) * L;
+    for (uint64_t t = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; t < total; t += uint64_t(gridDim.x) * blockDim.x) {
+        const uint32_t v = uint32_t(t / L), l = uint32_t(t % L);
+        deg0[t] = uint32_t(row_ptr[l][v + 1] - row_ptr[l][v]);
+    }
+}
+
+// Merged vertex-major adjacency; one wavefront per vertex.
+__global__ void k_build_adj(const int64_t* const* row_ptr, const uint32_t* const* col, const uint32_t* deg0,
+                            const int64_t* off, uint32_t n, uint32_t L, uint32_t* adj) {
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const uint32_t nwaves = gridDim.x * blockDim.x / kWave;
+    const uint32_t lane = lane_id();
+    for (uint32_t v = wave; v < n; v += nwaves) {
+        uint64_t dst = uint64_t(off[v]);
+        for (uint32_t l = 0; l < L; ++l) {
+            const int64_t s = row_ptr[l][v];
+            const uint32_t d = deg0[uint64_t(v) * L + l];
+            const uint32_t* src = col[l] + s;
+            for (uint32_t i = lane; i < d; i += kWave) adj[dst + i] = src[i] * L + l;
+            dst += d;
+        }
+    }
+}
+
+// Static chunk list (vertex, chunk#) covering all adjacency in kChunk pieces:
+// used by passes that visit every entry with its source vertex.
+__global__ void k_build_chunks(const int64_t* off, const uint64_t* chunk_off, uint32_t n, uint64_t* chunks) {
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+        const uint64_t b = chunk_off[v], e = chunk_off[v + 1];
+        for (uint64_t c = b; c < e; ++c) chunks[c] = uint64_t(v) | ((c - b) << 32);
+    }
+}
+
+// ------------------------------------------------------------ step control
+// One thread per instance.  s = step number.  Decides what step s does:
+// continue the level while the deferred list of parity s is non-empty,
+// finish when no vertex is alive, otherwise start the next level.
+__global__ void k_advance(InstState* states, Control* ctl, uint32_t I, uint32_t s) {
+    __shared__ int32_t s_active;
+    if (threadIdx.x == 0) s_active = 0;
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < I; i += blockDim.x) {
+        InstState* st = &states[i];
+        if (st->phase == kPhaseDone) continue;
+        if (s == 0) {
+            st->phase = kPhaseScan;
+            st->k = 0;
+        } else if (st->pcount[s & 1][0] > 0) {
+            st->phase = kPhaseCont;
+        } else if (st->alive[0] == 0) {
+            st->phase = kPhaseDone;
+            continue;
+        } else {
+            st->phase = kPhaseScan;
+            st->k += 1;
+        }
+        st->pcount[(s + 1) & 1][0] = 0;
+        atomicAdd(&s_active, 1);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) ctl->active = s_active;
+}
+
+// ------------------------------------------------------------------ scans
+struct ScanShared {
+    uint32_t keep;     // alive-list write position
+    uint32_t rem;      // ring write position
+    uint32_t removed;  // removals by this block (for the alive counter)
+};
+
+// Ring append of removed light vertices / deferral of heavy ones (scan side:
+// the ring holds at most cap entries and the scan removes at most cap).
+__device__ __forceinline__ void scan_emit(const Inst& in, InstState* st, Control* ctl, ScanShared* sh, uint32_t* ring,
+                                          uint32_t parity, bool keep, bool rem, uint32_t v, uint32_t* list) {
+    const uint64_t km = __ballot(keep);
+    if (km) {
+        uint32_t base = 0;
+        if (lane_id() == 0) base = atomicAdd(&sh->keep, uint32_t(__popcll(km)));
+        base = __shfl(base, 0, kWave);
+        if (keep) list[base + __builtin_amdgcn_mbcnt_hi(uint32_t(km >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(km), 0u))] = v;
+    }
+    bool heavy = false, light = false;
+    if (rem) {
+        const uint64_t d = uint64_t(in.off[v + 1] - in.off[v]);
+        heavy = d > kHeavy;
+        light = d > 0 && !heavy;
+    }
+    const uint64_t rm = __ballot(rem);
+    if (rm && lane_id() == 0) atomicAdd(&sh->removed, uint32_t(__popcll(rm)));
+    const uint64_t lm = __ballot(light);
+    if (lm) {
+        uint32_t base = 0;
+        if (lane_id() == 0) base = atomicAdd(&sh->rem, uint32_t(__popcll(lm)));
+        base = __shfl(base, 0, kWave);
+        if (light) ring[base + __builtin_amdgcn_mbcnt_hi(uint32_t(lm >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(lm), 0u))] = v;
+    }
+    defer_push(in, defer_list(in, parity), &st->pcount[parity][0], ctl, heavy, v);
+}
+
+// Level 0 for every instance in one pass over deg0: a vertex survives
+// instance lmd iff it has >= lmd layers of degree >= 1.  Survivors get their
+// degree row, cnt and a place on the block's alive list; the others get
+// index 0 and go to the ring (or the deferred list).
+__global__ void __launch_bounds__(kBlock) k_init(const Inst* insts, InstState* states, Control* ctl, uint32_t I,
+                                                 const uint32_t* deg0, uint32_t n, uint32_t L, uint32_t cap) {
+    __shared__ ScanShared sh[64];
+    for (uint32_t i = threadIdx.x; i < I; i += blockDim.x) sh[i] = ScanShared{0u, 0u, 0u};
+    __syncthreads();
+    const uint64_t lo = uint64_t(blockIdx.x) * cap;  // this block's list / ring slice
+    const uint32_t G = gridDim.x;
+    const uint32_t own = blockIdx.x < n ? uint32_t((uint64_t(n) - blockIdx.x + G - 1) / G) : 0u;  // b, b+G, ...
+    for (uint32_t base = 0; base < own; base += kBlock) {
+        const uint32_t t = base + threadIdx.x;
+        const bool valid = t < own;
+        const uint32_t v = valid ? blockIdx.x + t * G : 0u;
+        uint32_t nz = 0;
+        if (valid)
+            for (uint32_t l = 0; l < L; ++l) nz += deg0[uint64_t(v) * L + l] >= 1u;
+        if (valid) {
+            // instances lmd <= nz keep v: copy the row once per such instance
+            for (uint32_t l = 0; l < L; ++l) {
+                const uint32_t d = deg0[uint64_t(v) * L + l];
+                for (uint32_t i = 0; i < I && insts[i].lmd <= nz; ++i) insts[i].deg[uint64_t(v) * L + l] = d;
+            }
+        }
+        for (uint32_t i = 0; i < I; ++i) {
+            if (states[i].phase == kPhaseDone) continue;  // lmd above every vertex's layer count: all zero
+            const Inst& in = insts[i];
+            const bool alive = valid && nz >= in.lmd;
+            const bool dead = valid && !alive;
+            if (valid) {
+                in.idx[v] = alive ? -1 : 0;
+                if (alive) in.cnt[v] = int32_t(nz);
+            }
+            scan_emit(in, &states[i], ctl, &sh[i], in.ring + lo, 0u, alive, dead, v, in.list + lo);
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < I; i += blockDim.x) {
+        const Inst& in = insts[i];
+        if (states[i].phase == kPhaseDone) continue;
+        in.blk[2 * blockIdx.x] = sh[i].keep;
+        in.blk[2 * blockIdx.x + 1] = sh[i].rem;
+        if (sh[i].removed) atomicAdd(&states[i].alive[0], 0ull - (unsigned long long)sh[i].removed);
+    }
+}
+
+// Level k >= 1 scan of every instance in phase Scan: walk the block's alive
+// list, drop vertices removed since (idx >= 0), remove those with fewer than
+// lmd layers of degree >= k+1, keep the rest with their cnt.  The list is
+// compacted in place (writes never pass the chunk being read).
+__global__ void __launch_bounds__(kBlock) k_scan(const Inst* insts, InstState* states, Control* ctl, uint32_t I,
+                                                 uint32_t s) {
+    __shared__ ScanShared sh;
+    for (uint32_t i = 0; i < I; ++i) {
+        InstState* st = &states[i];
+        if (st->phase != kPhaseScan) continue;
+        const Inst& in = insts[i];
+        const int32_t k = st->k;
+        const uint32_t thr = uint32_t(k) + 1u;
+        const uint32_t L = in.L;
+        const uint64_t lo = uint64_t(blockIdx.x) * in.cap;
+        uint32_t* list = in.list + lo;
+        const uint32_t len = in.blk[2 * blockIdx.x];
+        if (threadIdx.x == 0) sh = ScanShared{0u, 0u, 0u};
+        __syncthreads();
+        for (uint32_t base = 0; base < len; base += kBlock) {
+            const uint32_t t = base + threadIdx.x;
+            uint32_t v = 0;
+            bool live = false;
+            if (t < len) {
+                v = list[t];
+                live = in.idx[v] < 0;
+            }
+            uint32_t c = 0;
+            if (live) {
+                const uint32_t* d = in.deg + uint64_t(v) * L;
+                for (uint32_t l = 0; l < L; ++l) c += d[l] >= thr;
+            }
+            const bool rem = live && c < in.lmd;
+            const bool keep = live && !rem;
+            if (rem) in.idx[v] = k;
+            if (keep) in.cnt[v] = int32_t(c);
+            __syncthreads();  // every entry of this chunk is read before compaction writes
+            scan_emit(in, st, ctl, &sh, in.ring + lo, s & 1u, keep, rem, v, list);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            in.blk[2 * blockIdx.x] = sh.keep;
+            in.blk[2 * blockIdx.x + 1] = sh.rem;
+            if (sh.removed) atomicAdd(&st->alive[0], 0ull - (unsigned long long)sh.removed);
+        }
+        __syncthreads();
+    }
+}
+
+// --------------------------------------------------------------- cascade
+// Decrements are combined per target entry in a block-private LDS hash
+// table and applied with one atomicSub(c) per distinct target: a hub that
+// loses many neighbours in one batch takes one global atomic instead of one
+// per neighbour.  The threshold crossing stays exact: the one subtraction
+// with old >= k+1 > old - c is the crossing one.
+constexpr uint32_t kAggSlots = 4096;
+constexpr uint32_t kAggEmpty = 0xFFFFFFFFu;  // entries are < n*L <= 2^32 - 1
+constexpr uint32_t kAggProbes = 8;
+
+struct CascShared {
+    uint32_t pre[kBlock];
+    uint64_t start[kBlock];
+    uint32_t key[kAggSlots];
+    uint32_t cnt[kAggSlots];
+    uint16_t slots[kAggSlots];  // slots holding a key, in insertion order
+    uint32_t tmp[kWavesPerBlock + 1];
+    uint32_t head;
+    uint32_t tail;
+    uint32_t removed;
+    uint32_t used;  // distinct keys in the table
+};
+
+// Places the vertices removed by this wavefront's lanes (rem) into the
+// block's ring (slots reserved by wave in LDS, never past head + cap) or,
+// when heavy or when the ring is full, into the deferred list of the next
+// step.  All lanes call.
+__device__ __forceinline__ void emit_removed(const Inst& in, InstState* st, Control* ctl, CascShared* sh,
+                                             uint32_t* ring, uint32_t next_parity, uint32_t head, bool rem,
+                                             uint32_t u) {
+    const uint64_t rm = __ballot(rem);
+    if (rm == 0) return;
+    if (lane_id() == 0) atomicAdd(&sh->removed, uint32_t(__popcll(rm)));
+    const uint32_t cap = in.cap;
+    uint64_t d = 0;
+    if (rem) d = uint64_t(in.off[u + 1] - in.off[u]);
+    const bool light = rem && d <= kHeavy;
+    const uint64_t lm = __ballot(light);
+    bool placed = false;
+    if (lm) {
+        const uint32_t want = uint32_t(__popcll(lm));
+        uint32_t base = 0xFFFFFFFFu;
+        if (lane_id() == 0) {
+            uint32_t t = sh->tail;
+            for (;;) {
+                if (t + want - head > cap) break;  // ring full: defer instead
+                const uint32_t prev = atomicCAS(&sh->tail, t, t + want);
+                if (prev == t) {
+                    base = t;
+                    break;
+                }
+                t = prev;
+            }
+        }
+        base = __shfl(base, 0, kWave);
+        if (base != 0xFFFFFFFFu) {
+            if (light) {
+                const uint32_t r = __builtin_amdgcn_mbcnt_hi(uint32_t(lm >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(lm), 0u));
+                ring[(base + r) % cap] = u;
+            }
+            placed = light;
+        }
+    }
+    defer_push(in, defer_list(in, next_parity), &st->pcount[next_parity][0], ctl, rem && !placed, u);
+}
+
+// Applies c decrements to deg[e] (u = e / L) and the cascade rule; returns
+// true when this call removed u.
+__device__ __forceinline__ bool apply_dec(const Inst& in, uint32_t e, uint32_t u, uint32_t c, uint32_t thr,
+                                          int32_t k) {
+    const uint32_t old = atomicSub(&in.deg[e], c);
+    if (old >= thr && old - c < thr) {
+        const int32_t oc = atomicSub(&in.cnt[u], 1);
+        if (oc == int32_t(in.lmd)) return atomicCAS(&in.idx[u], -1, k) == -1;
+    }
+    return false;
+}
+
+// Applies every combined decrement in the LDS table and empties it.  All
+// threads call.
+__device__ __forceinline__ void agg_flush(const Inst& in, InstState* st, Control* ctl, CascShared* sh, uint32_t* ring,
+                                          uint32_t next_parity, uint32_t head, uint32_t thr, int32_t k) {
+    __syncthreads();
+    const uint32_t used = sh->used;
+    for (uint32_t t0 = 0; t0 < used; t0 += kBlock) {
+        const uint32_t t = t0 + threadIdx.x;
+        bool rem = false;
+        uint32_t u = 0;
+        if (t < used) {
+            const uint32_t slot = sh->slots[t];
+            const uint32_t e = sh->key[slot];
+            const uint32_t c = sh->cnt[slot];
+            sh->key[slot] = kAggEmpty;
+            sh->cnt[slot] = 0;
+            u = div_L(e, in.L, in.lmagic);
+            rem = apply_dec(in, e, u, c, thr, k);
+        }
+        emit_removed(in, st, ctl, sh, ring, next_parity, head, rem, u);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) sh->used = 0;
+    __syncthreads();
+}
+
+// Expands the adjacency ranges (start, len) held by the block's threads
+// (len = 0 for none) and applies the cascade rule to every entry whose
+// target is alive.  All threads call; the LDS table must be empty on entry
+// and is empty on return.
+__device__ __forceinline__ void cascade_expand(const Inst& in, InstState* st, Control* ctl, CascShared* sh,
+                                               uint32_t* ring, uint32_t next_parity, int32_t k, uint64_t start,
+                                               uint32_t len) {
+    uint32_t total;
+    const uint32_t pre = block_excl_scan(len, &total, sh->tmp);
+    sh->pre[threadIdx.x] = pre;
+    sh->start[threadIdx.x] = start;
+    __syncthreads();
+    const uint32_t L = in.L, magic = in.lmagic;
+    const uint32_t thr = uint32_t(k) + 1u;
+    const uint32_t head = sh->head;
+    const bool agg = (in.flags & kFlagAggregate) != 0;
+    for (uint32_t j0 = 0; j0 < total; j0 += kBlock) {
+        const uint32_t j = j0 + threadIdx.x;
+        bool rem = false;
+        uint32_t u = 0;
+        if (j < total) {
+            uint32_t lo = 0, hi = kBlock - 1;  // largest p with pre[p] <= j
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi + 1) >> 1;
+                if (sh->pre[mid] <= j) lo = mid;
+                else hi = mid - 1;
+            }
+            const uint32_t e = in.adj[sh->start[lo] + (j - sh->pre[lo])];
+            u = div_L(e, L, magic);
+            if (in.idx[u] < 0) {
+                bool done = false;
+                if (agg) {  // combine in the LDS table; fall back to a direct atomic
+                    uint32_t h = (e * 2654435761u) >> 20;  // 12-bit hash
+                    for (uint32_t p = 0; p < kAggProbes && !done; ++p, h = (h + 1) & (kAggSlots - 1)) {
+                        uint32_t kk = sh->key[h];
+                        if (kk == kAggEmpty) {
+                            kk = atomicCAS(&sh->key[h], kAggEmpty, e);
+                            if (kk == kAggEmpty) {
+                                sh->slots[atomicAdd(&sh->used, 1u)] = uint16_t(h);
+                                kk = e;
+                            }
+                        }
+                        if (kk == e) {
+                            atomicAdd(&sh->cnt[h], 1u);
+                            done = true;
+                        }
+                    }
+                }
+                if (!done) rem = apply_dec(in, e, u, 1u, thr, k);
+            }
+        }
+        emit_removed(in, st, ctl, sh, ring, next_parity, head, rem, u);
+        if (agg) {
+            __syncthreads();
+            if (sh->used > kAggSlots / 2) agg_flush(in, st, ctl, sh, ring, next_parity, head, thr, k);
+        }
+    }
+    if (agg) agg_flush(in, st, ctl, sh, ring, next_parity, head, thr, k);
+    else __syncthreads();
+}
+
+// Cascade of every instance not done: deferred chunks of this step's list
+// (chunk j goes to block j mod G), then the block's ring until it is empty.
+__global__ void __launch_bounds__(kBlock) k_cascade(const Inst* insts, InstState* states, Control* ctl, uint32_t I,
+                                                    uint32_t s) {
+    __shared__ CascShared sh;
+    const uint32_t G = gridDim.x, b = blockIdx.x;
+    for (uint32_t t = threadIdx.x; t < kAggSlots; t += kBlock) {
+        sh.key[t] = kAggEmpty;
+        sh.cnt[t] = 0;
+    }
+    if (threadIdx.x == 0) sh.used = 0;
+    for (uint32_t i = 0; i < I; ++i) {
+        InstState* st = &states[i];
+        const int32_t phase = st->phase;
+        if (phase == kPhaseDone) continue;
+        const Inst& in = insts[i];
+        const int32_t k = st->k;
+        const uint32_t par = s & 1u, npar = par ^ 1u;
+        uint32_t* ring = in.ring + uint64_t(b) * in.cap;
+        if (threadIdx.x == 0) {
+            sh.head = 0;
+            sh.tail = phase == kPhaseScan ? in.blk[2 * b + 1] : 0u;
+            sh.removed = 0;
+        }
+        __syncthreads();
+        // deferred chunks
+        const uint64_t pc = st->pcount[par][0];
+        const uint64_t* pl = defer_list(in, par);
+        for (uint64_t m0 = 0; m0 * G + b < pc; m0 += kBlock) {
+            const uint64_t j = (m0 + threadIdx.x) * G + b;
+            uint64_t start = 0;
+            uint32_t len = 0;
+            if (j < pc) {
+                const uint64_t it = pl[j];
+                start = item_start(it);
+                len = item_len(it);
+            }
+            cascade_expand(in, st, ctl, &sh, ring, npar, k, start, len);
+        }
+        // ring
+        for (;;) {
+            const uint32_t head = sh.head, tail = sh.tail;
+            if (head >= tail) break;
+            const uint32_t nb = tail - head < uint32_t(kBlock) ? tail - head : uint32_t(kBlock);
+            uint64_t start = 0;
+            uint32_t len = 0;
+            if (threadIdx.x < nb) {
+                const uint32_t v = ring[(head + threadIdx.x) % in.cap];
+                start = uint64_t(in.off[v]);
+                len = uint32_t(uint64_t(in.off[v + 1]) - start);
+            }
+            cascade_expand(in, st, ctl, &sh, ring, npar, k, start, len);
+            if (threadIdx.x == 0) sh.head = head + nb;
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            in.blk[2 * b + 1] = 0;
+            if (sh.removed) atomicAdd(&st->alive[0], 0ull - (unsigned long long)sh.removed);
+        }
+        __syncthreads();
+    }
+}
+
+// Histogram of core indices per instance: hist[i*kcap + k] = #{v : idx_i(v) == k}.
+constexpr int kHistLds = 4096;
+__global__ void __launch_bounds__(kBlock) k_idx_hist(const int32_t* idx, uint32_t n, uint32_t I, uint64_t* hist,
+                                                     uint32_t kcap) {
+    __shared__ uint32_t s_h[kHistLds];
+    const uint32_t i = blockIdx.y;
+    const uint32_t lds_k = kcap < uint32_t(kHistLds) ? kcap : uint32_t(kHistLds);
+    for (uint32_t t = threadIdx.x; t < lds_k; t += blockDim.x) s_h[t] = 0;
+    __syncthreads();
+    const int32_t* row = idx + uint64_t(i) * n;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+        const int32_t x = row[v];
+        const uint32_t k = x < 0 ? 0u : uint32_t(x);  // every vertex has x >= 0 after a decomposition
+        if (k < lds_k) atomicAdd(&s_h[k], 1u);
+        else if (k < kcap) atomicAdd(reinterpret_cast<unsigned long long*>(&hist[uint64_t(i) * kcap + k]), 1ull);
+    }
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < lds_k; t += blockDim.x)
+        if (s_h[t]) atomicAdd(reinterpret_cast<unsigned long long*>(&hist[uint64_t(i) * kcap + t]), (unsigned long long)s_h[t]);
+}
+
+// ------------------------------------------------------- densest subgraph
+// Edge histogram of one instance: for every directed entry v -> (u, l) with
+// m = min(idx[v], idx[u]) >= 1, hist[l][m] += 1.  Then num_edge_l(k) =
+// sum_{m >= k} hist[l][m] = sum over v in S(k) of |N_l(v) ∩ S(k)|.
+// Block-private LDS bins for small m, global atomics above.
+constexpr int kLdsBins = 8192;
+constexpr int kHistBlock = 256;
+constexpr int kHistWaves = kHistBlock / kWave;
+
+__global__ void __launch_bounds__(kHistBlock) k_edge_hist(const Inst in, const uint64_t* chunks, uint64_t nchunks,
+                                                          uint64_t* hist, uint32_t kcap) {
+    __shared__ uint32_t s_hist[kLdsBins];
+    __shared__ uint32_t s_pre_all[kHistWaves][kWave];
+    __shared__ uint64_t s_start_all[kHistWaves][kWave];
+    __shared__ uint32_t s_src_all[kHistWaves][kWave];
+    const uint32_t L = in.L;
+    const uint32_t lds_k = kLdsBins / L < kcap ? kLdsBins / L : kcap;  // bins m < lds_k live in LDS
+    for (uint32_t t = threadIdx.x; t < kLdsBins; t += blockDim.x) s_hist[t] = 0;
+    __syncthreads();
+    const uint32_t lane = lane_id();
+    const uint32_t wib = threadIdx.x / kWave;
+    uint32_t* s_pre = s_pre_all[wib];
+    uint64_t* s_start = s_start_all[wib];
+    uint32_t* s_src = s_src_all[wib];
+    const uint64_t gwave = blockIdx.x * uint64_t(kHistWaves) + wib;
+    const uint64_t nwaves = uint64_t(gridDim.x) * kHistWaves;
+    for (uint64_t c0 = gwave * kWave; c0 < nchunks; c0 += nwaves * kWave) {
+        const uint64_t c = c0 + lane;
+        uint32_t len = 0, v = 0;
+        uint64_t s = 0;
+        if (c < nchunks) {
+            const uint64_t ch = chunks[c];
+            v = uint32_t(ch);
+            const uint64_t ci = ch >> 32;
+            const uint64_t b = uint64_t(in.off[v]), e = uint64_t(in.off[v + 1]);
+            s = b + ci * kChunk;
+            len = uint32_t((e - s) < uint64_t(kChunk) ? (e - s) : uint64_t(kChunk));
+        }
+        const int32_t iv = len ? in.idx[v] : 0;
+        if (iv < 1) len = 0;  // v outside S(1): contributes to no cell
+        uint32_t total;
+        const uint32_t pre = wave_excl_scan(len, &total);
+        s_pre[lane] = pre;
+        s_start[lane] = s;
+        s_src[lane] = uint32_t(iv);
+        __builtin_amdgcn_wave_barrier();
+        for (uint32_t j0 = 0; j0 < total; j0 += kWave) {
+            const uint32_t j = j0 + lane;
+            if (j < total) {
+                uint32_t lo = 0, hi = kWave - 1;
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi + 1) >> 1;
+                    if (s_pre[mid] <= j) lo = mid;
+                    else hi = mid - 1;
+                }
+                const uint32_t e = in.adj[s_start[lo] + (j - s_pre[lo])];
+                const uint32_t u = div_L(e, L, in.lmagic);
+                const uint32_t l = e - u * L;
+                const int32_t iu = in.idx[u];
+                const uint32_t m = uint32_t(iu < int32_t(s_src[lo]) ? iu : int32_t(s_src[lo]));
+                if (iu >= 1) {
+                    if (m < lds_k) atomicAdd(&s_hist[l * lds_k + m], 1u);
+                    else atomicAdd(reinterpret_cast<unsigned long long*>(&hist[uint64_t(l) * kcap + m]), 1ull);
+                }
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < lds_k * L; t += blockDim.x) {
+        const uint32_t x = s_hist[t];
+        if (x) {
+            const uint32_t l = t / lds_k, m = t % lds_k;
+            atomicAdd(reinterpret_cast<unsigned long long*>(&hist[uint64_t(l) * kcap + m]), (unsigned long long)x);
+        }
+    }
+}
+
+// One thread per cell (k, lmd), 1 <= k <= kmax: FCPathLevelLeft::constructCore's
+// wds block (Core/FCPathLevelLeft.cpp:57-99) on suffix sums of the histogram.
+// hist rows are [l][m] per instance; cell_len[k] = |S(k)|.
+__global__ void k_cell_score(const uint64_t* hist, uint32_t kcap, uint32_t L, uint32_t kmax, const int64_t* cell_len,
+                             const double* pow_tab, double* density, int32_t* sel) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    if (k > kmax) return;
+    double dl[64];
+    for (uint32_t l = 0; l < L; ++l) {
+        uint64_t s = 0;
+        for (uint32_t m = k; m <= kmax; ++m) s += hist[uint64_t(l) * kcap + m];
+        const uint32_t num_edge = uint32_t(s);  // uint32 accumulation as the reference
+        dl[l] = double(float(num_edge) * 1.0f / 2);
+    }
+    // insertion sort, descending
+    for (uint32_t a = 1; a < L; ++a) {
+        const double x = dl[a];
+        uint32_t b = a;
+        while (b > 0 && dl[b - 1] < x) {
+            dl[b] = dl[b - 1];
+            --b;
+        }
+        dl[b] = x;
+    }
+    const double length = double(uint32_t(cell_len[k]));
+    double best = 0.0;
+    int32_t bsel = 0;
+    for (uint32_t l = 0; l < L; ++l) {
+        const double mad = (dl[l] * 1.0f) / length * 1.0;
+        const double dens = mad * pow_tab[l];
+        if (dens >= best) {
+            best = dens;
+            bsel = int32_t(l + 1);
+        }
+    }
+    density[k] = best;
+    sel[k] = bsel;
+}
+
+}  // namespace hg

@@ -1,0 +1,183 @@
+"""GPU parity tests: libhipgrid.so through its C ABI against the oracle.
+
+Bar: bit-exact core indices, cell lengths, k-shells and WdsList results.
+Full-size workloads are checked through size-independent properties in
+test_gpu_properties.py; here the oracle finishes in seconds.
+"""
+import dataclasses
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from pargridcpu_amd.graph import C1, MultilayerGraph
+from pargridcpu_amd.hipgrid import Engine, HgError, HipGrid
+from tests import graphs as G
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg: str) -> None:
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def small_spec(n, L, m, seed, gamma=2.3, rho=0.5):
+    return dataclasses.replace(C1, name=f"t{n}-{L}-{m}-{seed}", seed=seed, n=n, L=L, m_lo=m, m_hi=m, gamma=gamma,
+                               rho=rho, order=0)
+
+
+def run_engine(mg):
+    with Engine.from_graph(mg) as eng:
+        eng.decompose()
+        ci = eng.core_index()
+        cl, kmax = eng.cells()
+        t = eng.timing()
+    return ci, cl, kmax, t
+
+
+def check_against_oracle(mg, ci, cl, kmax):
+    ref = O.coreindex(mg.n, mg.layers())
+    assert np.array_equal(ci, ref)
+    assert np.array_equal(kmax, ref.max(axis=1))
+    for i in range(mg.L):
+        for k in range(cl.shape[1]):
+            assert cl[i, k] == (np.count_nonzero(ref[i] >= k) if k <= kmax[i] else 0)
+
+
+@pytest.mark.parametrize("n,L,m,seed", [(50, 1, 120, 1), (200, 3, 600, 2), (1000, 4, 3000, 3), (3000, 7, 8000, 4),
+                                        (2000, 12, 2500, 5), (500, 2, 20000, 6)])
+def test_seeded_small(n, L, m, seed):
+    mg = MultilayerGraph.generate(small_spec(n, L, m, seed))
+    say(f"case seeded n={n} L={L} m={m} seed={seed}")
+    ci, cl, kmax, t = run_engine(mg)
+    check_against_oracle(mg, ci, cl, kmax)
+    # ParGrid restatement agrees as well (cells from the reference flow)
+    pci, pcl, pk = O.pargrid(mg.n, mg.layers(), cap=cl.shape[1])
+    assert np.array_equal(ci, pci)
+    assert np.array_equal(cl, pcl)
+
+
+def test_c1_workload():
+    mg = MultilayerGraph.generate(C1)
+    say("case C1 n=20000 L=7")
+    ci, cl, kmax, t = run_engine(mg)
+    check_against_oracle(mg, ci, cl, kmax)
+    assert t.levels >= int(kmax.max()) + 1
+
+
+def test_closed_forms():
+    n = 40
+    # identical layers of a 6-clique + star + cycle + path: index = k-core number for every lmd
+    es = G.clique(range(6)) + G.star(6, range(7, 15)) + G.cycle(range(15, 25)) + G.path(range(25, 33))
+    mg = G.graph(n, [es, es, es])
+    say("case closed forms")
+    ci, cl, kmax, _ = run_engine(mg)
+    kc = G.kcore_numbers(n, es)
+    for i in range(3):
+        assert np.array_equal(ci[i], kc)
+    assert list(ci[0][:6]) == [5] * 6
+    assert ci[0][6] == 1 and all(ci[0][7:15] == 1)
+    assert all(ci[0][15:25] == 2) and all(ci[0][25:33] == 1) and all(ci[0][33:] == 0)
+    # L = 1 is the classic k-core decomposition
+    rng = np.random.default_rng(7)
+    es1 = [tuple(x) for x in rng.integers(0, 300, size=(1500, 2))]
+    mg1 = G.graph(300, [es1])
+    ci1, _, _, _ = run_engine(mg1)
+    assert np.array_equal(ci1[0], G.kcore_numbers(300, es1))
+
+
+def test_edge_cases():
+    say("case no edges")
+    mg = G.graph(10, [[], []])
+    ci, cl, kmax, _ = run_engine(mg)
+    assert (ci == 0).all() and (kmax == 0).all()
+    say("case single edge, one layer of two")
+    mg = G.graph(3, [[(0, 1)], []])
+    ci, cl, kmax, _ = run_engine(mg)
+    assert ci.tolist() == [[1, 1, 0], [0, 0, 0]]
+    say("case one vertex")
+    mg = G.graph(1, [[]])
+    ci, _, _, _ = run_engine(mg)
+    assert ci.tolist() == [[0]]
+    say("case hub (chunk splitting)")
+    hub_edges = G.star(0, range(1, 5000)) + G.cycle(range(1, 5000))
+    mg = G.graph(5000, [hub_edges, hub_edges[::2]])
+    ci, cl, kmax, _ = run_engine(mg)
+    check_against_oracle(mg, ci, cl, kmax)
+
+
+def test_repeat_and_reuse():
+    mg = MultilayerGraph.generate(small_spec(4000, 5, 12000, 11))
+    say("case repeat decompose x3")
+    with Engine.from_graph(mg) as eng:
+        outs = []
+        for _ in range(3):
+            eng.decompose()
+            outs.append(eng.core_index())
+    assert all(np.array_equal(outs[0], o) for o in outs[1:])
+    check_against_oracle(mg, outs[0], *Engine_cells(mg))
+
+
+def Engine_cells(mg):
+    with Engine.from_graph(mg) as eng:
+        eng.decompose()
+        return eng.cells()
+
+
+def test_shells():
+    mg = MultilayerGraph.generate(small_spec(2500, 4, 7000, 12))
+    say("case shells")
+    with Engine.from_graph(mg) as eng:
+        eng.decompose()
+        ci = eng.core_index()
+        off, vtx, kcap = eng.shells()
+    stride = kcap + 1
+    for i in range(mg.L):
+        for k in range(kcap):
+            a, b = off[i * stride + k], off[i * stride + k + 1]
+            expect = np.nonzero(ci[i] == k)[0]
+            assert np.array_equal(vtx[a:b], expect.astype(np.uint32))
+
+
+@pytest.mark.parametrize("beta", [1.0, 0.5, 2.0])
+def test_densest(beta):
+    mg = MultilayerGraph.generate(small_spec(3000, 6, 9000, 13))
+    say(f"case densest beta={beta}")
+    ref = O.wds(mg.n, mg.layers(), beta)
+    with Engine.from_graph(mg) as eng:
+        eng.decompose()
+        w = eng.densest(beta)
+        cap = ref["cell_density"].shape[1]
+        dens, sel = eng.cell_density(cap)
+        verts = eng.densest_vertices()
+        ci = eng.core_index()
+    assert np.float32(w.max_density) == ref["max_density"]
+    assert (w.res_k, w.res_lmd, w.res_len) == (ref["res_k"], ref["res_lmd"], ref["res_len"])
+    assert np.array_equal(dens, ref["cell_density"])
+    assert np.array_equal(sel, ref["cell_sel"])
+    assert np.array_equal(verts, np.nonzero(ci[w.res_lmd - 1] >= w.res_k)[0].astype(np.uint32))
+    assert verts.size == w.res_len
+
+
+def test_densest_c1():
+    mg = MultilayerGraph.generate(C1)
+    say("case densest C1")
+    ref = O.wds(mg.n, mg.layers(), 1.0)
+    m, k, lmd, ln, verts = HipGrid.WeightDenestSubgraph(mg, 1.0)
+    assert (m, k, lmd, ln) == (ref["max_density"], ref["res_k"], ref["res_lmd"], ref["res_len"])
+
+
+def test_errors():
+    say("case errors")
+    mg = G.graph(4, [[(0, 1)]])
+    with pytest.raises(HgError) as e:
+        Engine(mg.layers(), 4, offset_bits=16)
+    assert e.value.code == -1
+    with Engine.from_graph(mg) as eng:
+        with pytest.raises(HgError) as e:
+            eng.core_index()
+        assert e.value.code == -5
+        with pytest.raises(HgError):
+            eng.densest(1.0)
