@@ -101,6 +101,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default=os.environ.get("HG_WORKLOAD", "c2"))
     ap.add_argument("--cpu-baseline", default="auto", choices=["auto", "reference", "port", "off"])
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="also run one device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the lambda-sharded multi-GPU "
@@ -123,6 +125,13 @@ def main() -> int:
     eng = Engine.from_graph(g, stream=stream.cuda_stream)
     upload_s = time.perf_counter() - t0
     log(f"[bench] uploaded n={n} L={L} D={D} in {upload_s:.2f}s ({eng.info()['device_bytes'] / 2**30:.1f} GiB)")
+    if args.calib_copy_mib:
+        a = torch.empty(args.calib_copy_mib << 20, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"[bench] calibration copy: {args.calib_copy_mib} MiB read + {args.calib_copy_mib} MiB written")
+        del a, b
     for i in range(args.warmup):
         eng.decompose()
         log(f"[bench] warmup {i}: {eng.timing().decompose_ms:.3f} ms")

@@ -2,12 +2,17 @@
 //
 // Device memory layout of one graph (n vertices, L layers, D directed
 // entries over all layers):
+//   (internal vertex ids: caller's ids sorted by total degree, descending)
 //   off     int64 [n+1]       vertex-major adjacency offsets
 //   adj     uint32 [D]        entry u*L + l (neighbour u in layer l)
+//   adjz    uint8 [D]         z(u) of each entry's target
 //   deg0    uint32 [n][L]     initial layer degrees
+//   z       uint8 [n]         number of layers in which the vertex has an edge
+//   inv     uint32 [n]        caller's id -> internal id
 //   chunks  uint64 [nchunks]  (v, chunk#) list covering adj in kChunk pieces
+//   out     int32 [L][n]      core index in the caller's ids (the result)
 // and per instance lmd = 1..L (all allocated once, reused by every call):
-//   deg uint32 [n][L], cnt int32 [n], idx int32 [n] (the core index row),
+//   deg uint32 [n][L], cnt int32 [n], idx int32 [n] (internal ids),
 //   per-block alive list and removal ring uint32 [G*cap] each (block b of
 //   the fixed grid owns vertices [b*cap, (b+1)*cap)), two deferred chunk
 //   lists uint64 [pcap] (pcap = sum over v of ceil(deg(v) / kDefer)).
@@ -76,7 +81,14 @@ struct hg_graph {
     // graph
     int64_t* off = nullptr;
     uint32_t* adj = nullptr;
+    uint8_t* adjz = nullptr;
     uint32_t* deg0 = nullptr;
+    uint32_t* inv = nullptr;
+    uint8_t* z = nullptr;
+    uint32_t* alive = nullptr;  // per instance ceil(n/32) words
+    uint32_t H = 0;             // vertices (lowest internal ids) pulled by a wavefront each
+    uint32_t pull = 0;          // level 0 by pull (L <= kPullMaxL)
+    int32_t* out = nullptr;
     uint64_t* chunks = nullptr;
     // instances
     uint32_t* deg = nullptr;
@@ -115,7 +127,7 @@ namespace {
 void release(hg_graph* g) {
     if (!g) return;
     (void)hipSetDevice(g->device);
-    void* ptrs[] = {g->off, g->adj, g->deg0, g->chunks, g->deg, g->cnt, g->idx, g->lists, g->rings, g->blk, g->pl,
+    void* ptrs[] = {g->off, g->adj, g->adjz, g->deg0, g->alive, g->inv, g->z, g->out, g->chunks, g->deg, g->cnt, g->idx, g->lists, g->rings, g->blk, g->pl,
                     g->d_ihist, g->d_insts, g->d_states, g->d_ctl, g->hist, g->d_density, g->d_sel, g->d_cell_len,
                     g->d_pow};
     for (void* p : ptrs)
@@ -163,42 +175,80 @@ int create_impl(const hg_layer_csr* layers, int32_t L, int64_t n, int32_t offset
             maxdeg = std::max<uint64_t>(maxdeg, uint64_t(dst[v + 1] - dst[v]));
         }
     }
+    // Totals and non-empty layer counts in the caller's ids, then the
+    // relabelling: internal id = rank by total degree, descending (counting
+    // sort, stable in the caller's id).
+    std::vector<uint64_t> tot(static_cast<size_t>(n));
+    std::vector<uint8_t> nz_c(static_cast<size_t>(n));
+    uint64_t maxtot = 0;
+    for (int64_t v = 0; v < n; ++v) {
+        uint64_t t = 0;
+        uint32_t z = 0;
+        for (int l = 0; l < L; ++l) {
+            const int64_t d = rp[size_t(l)][size_t(v) + 1] - rp[size_t(l)][size_t(v)];
+            t += uint64_t(d);
+            z += d > 0;
+        }
+        tot[size_t(v)] = t;
+        nz_c[size_t(v)] = uint8_t(z);
+        maxtot = std::max(maxtot, t);
+    }
+    std::vector<uint32_t> perm(static_cast<size_t>(n)), inv(static_cast<size_t>(n));
+    {
+        std::vector<uint64_t> bucket(size_t(maxtot) + 2, 0);
+        for (int64_t v = 0; v < n; ++v) bucket[size_t(maxtot - tot[size_t(v)]) + 1]++;
+        for (size_t b = 1; b < bucket.size(); ++b) bucket[b] += bucket[b - 1];
+        for (int64_t v = 0; v < n; ++v) {
+            const uint32_t r = uint32_t(bucket[size_t(maxtot - tot[size_t(v)])]++);
+            perm[r] = uint32_t(v);
+            inv[size_t(v)] = r;
+        }
+    }
     std::vector<int64_t> off(size_t(n) + 1);
     std::vector<uint64_t> coff(size_t(n) + 1);
+    std::vector<uint8_t> zr(static_cast<size_t>(n));
+    uint32_t H = 0;
     off[0] = 0;
     coff[0] = 0;
     uint64_t pcap = 0;
     uint32_t max_nz = 0;
-    for (int64_t v = 0; v < n; ++v) {
-        int64_t t = 0;
-        uint32_t nz = 0;
-        for (int l = 0; l < L; ++l) {
-            const int64_t d = rp[size_t(l)][size_t(v) + 1] - rp[size_t(l)][size_t(v)];
-            t += d;
-            nz += d > 0;
-        }
-        max_nz = std::max(max_nz, nz);
-        off[size_t(v) + 1] = off[size_t(v)] + t;
-        coff[size_t(v) + 1] = coff[size_t(v)] + uint64_t((t + kChunk - 1) / kChunk);
-        pcap += uint64_t((t + kDefer - 1) / kDefer);
+    for (int64_t r = 0; r < n; ++r) {
+        const uint32_t v = perm[size_t(r)];
+        const uint64_t t = tot[size_t(v)];
+        zr[size_t(r)] = nz_c[v];
+        if (t > kPullHeavy) H = uint32_t(r) + 1;  // totals descend with r
+        max_nz = std::max<uint32_t>(max_nz, nz_c[v]);
+        off[size_t(r) + 1] = off[size_t(r)] + int64_t(t);
+        coff[size_t(r) + 1] = coff[size_t(r)] + (t + kChunk - 1) / kChunk;
+        pcap += (t + kDefer - 1) / kDefer;
     }
+    std::vector<uint64_t>().swap(tot);
     g->D = uint64_t(off[size_t(n)]);
     if (g->D >= (1ull << 40)) return fail(HG_ERR_RANGE, "total directed entries must be < 2^40");
     g->nchunks = coff[size_t(n)];
     g->pcap = pcap + 1;
     g->max_nz = max_nz;
+    g->H = H;
+    g->pull = uint32_t(L) <= kPullMaxL && env_int("HG_PULL", 1) == 1 ? 1u : 0u;
     g->kcap = uint32_t(maxdeg + 2);
     // Level-kernel grid: HG_BLOCKS overrides 2 blocks per CU.
     const uint32_t want_g = uint32_t(env_int("HG_BLOCKS", 2 * g->cus));
     g->G = std::max(1u, std::min<uint32_t>(want_g, uint32_t(n)));
-    g->cap = uint32_t((uint64_t(n) + g->G - 1) / g->G);
+    // list slice per block: its runs of kRun ids, dealt round robin
+    g->cap = uint32_t(((uint64_t(n) + kRun - 1) / kRun + g->G - 1) / g->G * kRun);
 
     const size_t nL = size_t(n) * size_t(L);
     int64_t bytes = 0;
     auto acct = [&](size_t b) { bytes += int64_t(b); };
     HIP_TRY(dalloc(&g->off, size_t(n) + 1)); acct(8 * (size_t(n) + 1));
     HIP_TRY(dalloc(&g->adj, g->D)); acct(4 * g->D);
+    HIP_TRY(dalloc(&g->adjz, g->D)); acct(g->D);
     HIP_TRY(dalloc(&g->deg0, nL)); acct(4 * nL);
+    const size_t aw = (size_t(n) + 31) / 32;
+    HIP_TRY(dalloc(&g->alive, aw * L)); acct(4 * aw * L);
+    HIP_TRY(dalloc(&g->inv, size_t(n))); acct(4 * size_t(n));
+    HIP_TRY(dalloc(&g->z, size_t(n))); acct(size_t(n));
+    HIP_TRY(dalloc(&g->out, nL)); acct(4 * nL);
     HIP_TRY(dalloc(&g->chunks, g->nchunks)); acct(8 * g->nchunks);
     HIP_TRY(dalloc(&g->deg, nL * L)); acct(4 * nL * L);
     HIP_TRY(dalloc(&g->cnt, nL)); acct(4 * nL);
@@ -226,7 +276,11 @@ int create_impl(const hg_layer_csr* layers, int32_t L, int64_t n, int32_t offset
     int64_t** d_rp_arr = nullptr;
     uint32_t** d_col_arr = nullptr;
     uint64_t* d_coff = nullptr;
+    uint32_t* d_perm = nullptr;
+    uint8_t* d_zc = nullptr;
     auto cleanup = [&]() {
+        if (d_perm) (void)hipFree(d_perm);
+        if (d_zc) (void)hipFree(d_zc);
         for (auto p : d_rp) if (p) (void)hipFree(p);
         for (auto p : d_col) if (p) (void)hipFree(p);
         if (d_rp_arr) (void)hipFree(d_rp_arr);
@@ -248,6 +302,15 @@ int create_impl(const hg_layer_csr* layers, int32_t L, int64_t n, int32_t offset
     if (e == hipSuccess) e = dalloc(&d_rp_arr, size_t(L));
     if (e == hipSuccess) e = dalloc(&d_col_arr, size_t(L));
     if (e == hipSuccess) e = dalloc(&d_coff, size_t(n) + 1);
+    if (e == hipSuccess) e = dalloc(&d_perm, size_t(n));
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d_perm, perm.data(), sizeof(uint32_t) * size_t(n), hipMemcpyHostToDevice, g->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(g->inv, inv.data(), sizeof(uint32_t) * size_t(n), hipMemcpyHostToDevice, g->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(g->z, zr.data(), size_t(n), hipMemcpyHostToDevice, g->stream);
+    if (e == hipSuccess) e = dalloc(&d_zc, size_t(n));
+    if (e == hipSuccess) e = hipMemcpyAsync(d_zc, nz_c.data(), size_t(n), hipMemcpyHostToDevice, g->stream);
+
     if (e == hipSuccess)
         e = hipMemcpyAsync(d_rp_arr, d_rp.data(), sizeof(int64_t*) * L, hipMemcpyHostToDevice, g->stream);
     if (e == hipSuccess)
@@ -258,11 +321,12 @@ int create_impl(const hg_layer_csr* layers, int32_t L, int64_t n, int32_t offset
         e = hipMemcpyAsync(d_coff, coff.data(), sizeof(uint64_t) * (size_t(n) + 1), hipMemcpyHostToDevice, g->stream);
     if (e == hipSuccess) {
         const uint32_t grid = uint32_t(std::min<uint64_t>((nL + kBlock - 1) / kBlock, uint64_t(g->cus) * 16));
-        hipLaunchKernelGGL(k_build_deg0, dim3(grid), dim3(kBlock), 0, g->stream, d_rp_arr, g->n, g->L, g->deg0);
+        hipLaunchKernelGGL(k_build_deg0, dim3(grid), dim3(kBlock), 0, g->stream, d_rp_arr, d_perm, g->n, g->L,
+                           g->deg0);
         const uint32_t grid2 = uint32_t(std::min<uint64_t>((uint64_t(n) * kWave + kBlock - 1) / kBlock,
                                                            uint64_t(g->cus) * 16));
         hipLaunchKernelGGL(k_build_adj, dim3(grid2), dim3(kBlock), 0, g->stream, d_rp_arr, d_col_arr, g->deg0,
-                           g->off, g->n, g->L, g->adj);
+                           g->off, d_perm, g->inv, d_zc, g->n, g->L, g->adj, g->adjz);
         const uint32_t grid3 = uint32_t(std::min<uint64_t>((uint64_t(n) + kBlock - 1) / kBlock, uint64_t(g->cus) * 16));
         hipLaunchKernelGGL(k_build_chunks, dim3(grid3), dim3(kBlock), 0, g->stream, g->off, d_coff, g->n, g->chunks);
         e = hipGetLastError();
@@ -278,6 +342,8 @@ int create_impl(const hg_layer_csr* layers, int32_t L, int64_t n, int32_t offset
         Inst& in = g->insts[i];
         in.off = g->off;
         in.adj = g->adj;
+        in.adjz = g->adjz;
+        in.alive = g->alive + aw * i;
         in.deg = g->deg + size_t(i) * nL;
         in.cnt = g->cnt + size_t(i) * n;
         in.idx = g->idx + size_t(i) * n;
@@ -315,9 +381,16 @@ const char* err_name(int32_t e) {
 void launch_step(hg_graph* g, uint32_t s) {
     const uint32_t L = g->L;
     hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, g->stream, g->d_states, g->d_ctl, L, s);
-    if (s == 0)
+    if (s == 0) {
+        if (g->pull && g->H > 0) {
+            const uint32_t gh = uint32_t(std::min<uint64_t>((uint64_t(g->H) * kWave + kBlock - 1) / kBlock,
+                                                            uint64_t(g->cus) * 8));
+            hipLaunchKernelGGL(k_pull_heavy, dim3(gh), dim3(kBlock), 0, g->stream, g->d_insts, L, g->deg0, g->z, g->H,
+                               g->L);
+        }
         hipLaunchKernelGGL(k_init, dim3(g->G), dim3(kBlock), 0, g->stream, g->d_insts, g->d_states, g->d_ctl, L,
-                           g->deg0, g->n, g->L, g->cap);
+                           g->deg0, g->z, g->n, g->L, g->cap, g->H, g->pull);
+    }
     else
         hipLaunchKernelGGL(k_scan, dim3(g->G), dim3(kBlock), 0, g->stream, g->d_insts, g->d_states, g->d_ctl, L, s);
     hipLaunchKernelGGL(k_cascade, dim3(g->G), dim3(kBlock), 0, g->stream, g->d_insts, g->d_states, g->d_ctl, L, s);
@@ -333,10 +406,10 @@ int decompose_impl(hg_graph* g) {
     for (uint32_t i = 0; i < L; ++i) {
         st[i].alive[0] = g->n;
         if (i + 1 > g->max_nz) {
-            // No vertex has edges in lmd layers: S(1, lmd) is empty, every index is 0.
+            // No vertex has edges in lmd layers: S(1, lmd) is empty, every
+            // index is 0 (k_init writes the zeros).
             st[i].phase = kPhaseDone;
             st[i].alive[0] = 0;
-            HIP_TRY(hipMemsetAsync(g->insts[i].idx, 0, sizeof(int32_t) * g->n, g->stream));
         }
     }
     Control ctl;
@@ -369,7 +442,10 @@ int decompose_impl(hg_graph* g) {
     // Cell lengths and kmax from the histogram of core indices.
     HIP_TRY(hipMemsetAsync(g->d_ihist, 0, sizeof(uint64_t) * size_t(g->kcap) * L, g->stream));
     const uint32_t hx = uint32_t(std::min<uint64_t>((uint64_t(g->n) + kBlock - 1) / kBlock, uint64_t(g->cus) * 2));
-    hipLaunchKernelGGL(k_idx_hist, dim3(std::max(hx, 1u), L), dim3(kBlock), 0, g->stream, g->idx, g->n, L, g->d_ihist,
+    const uint32_t ux = uint32_t(std::min<uint64_t>((uint64_t(g->n) * L + kBlock - 1) / kBlock, uint64_t(g->cus) * 16));
+    hipLaunchKernelGGL(k_unpermute, dim3(std::max(ux, 1u)), dim3(kBlock), 0, g->stream, g->d_insts, g->inv, g->n, L,
+                       g->out);
+    hipLaunchKernelGGL(k_idx_hist, dim3(std::max(hx, 1u), L), dim3(kBlock), 0, g->stream, g->out, g->n, L, g->d_ihist,
                        g->kcap);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(g->ev[2], g->stream));
@@ -523,7 +599,7 @@ int hg_core_index(hg_graph* g, int32_t* core_index) {
     if (!g || !core_index) return fail(HG_ERR_ARG, "NULL argument");
     if (!g->decomposed) return fail(HG_ERR_STATE, "hg_decompose has not completed");
     HIP_TRY(hipSetDevice(g->device));
-    HIP_TRY(hipMemcpyAsync(core_index, g->idx, sizeof(int32_t) * size_t(g->n) * g->L, hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipMemcpyAsync(core_index, g->out, sizeof(int32_t) * size_t(g->n) * g->L, hipMemcpyDeviceToHost, g->stream));
     HIP_TRY(hipStreamSynchronize(g->stream));
     return HG_OK;
 }
@@ -531,7 +607,7 @@ int hg_core_index(hg_graph* g, int32_t* core_index) {
 int hg_core_index_device(hg_graph* g, const int32_t** dev_ptr) {
     if (!g || !dev_ptr) return fail(HG_ERR_ARG, "NULL argument");
     if (!g->decomposed) return fail(HG_ERR_STATE, "hg_decompose has not completed");
-    *dev_ptr = g->idx;
+    *dev_ptr = g->out;
     return HG_OK;
 }
 
@@ -607,7 +683,7 @@ int hg_densest_vertices(hg_graph* g, uint32_t* out, int64_t cap, int64_t* count)
     try {
         std::vector<int32_t> row(g->n);
         HIP_TRY(hipSetDevice(g->device));
-        HIP_TRY(hipMemcpyAsync(row.data(), g->idx + size_t(g->wds.res_lmd - 1) * g->n, sizeof(int32_t) * g->n,
+        HIP_TRY(hipMemcpyAsync(row.data(), g->out + size_t(g->wds.res_lmd - 1) * g->n, sizeof(int32_t) * g->n,
                                hipMemcpyDeviceToHost, g->stream));
         HIP_TRY(hipStreamSynchronize(g->stream));
         int64_t c = 0;

@@ -4,7 +4,10 @@
 // graph resident in HBM.  Per instance and vertex: the L layer degrees in
 // the residual graph (deg, AoS [n][L]: one vertex's row is one or two
 // 32-byte sectors), cnt = number of layers with degree >= the current
-// threshold, and idx = core index (-1 while alive).  Level k removes every
+// threshold, and idx = core index (-1 while alive).  Separate compact
+// arrays: the random accesses are bound by cache capacity, so the hot
+// footprint per vertex is kept small (a 64-byte record per vertex measured
+// 1.5x slower on the 1.2B-edge workload).  Level k removes every
 // alive vertex whose index is k, turning S(k, lmd) into S(k+1, lmd).  This
 // is FCSyncLeft::PeelSync's two-phase rule (CoreParallel/FCSyncLeft.cpp:
 // 65-108):
@@ -18,17 +21,28 @@
 // Scheduling, designed around the cost of same-address device atomics
 // (they serialise, ~11 ns each on MI355X, while scattered atomics reach
 // ~26 G/s):
-//   * The grid is G fixed blocks.  Block b owns the vertices v = b mod G for
-//     the scan (strided, so the hubs of a power-law graph, which cluster at
-//     low ids in the generator, spread over all blocks) and keeps their
-//     alive list compacted in place in its own list slice: no global
-//     counters.
-//   * Removals go to the discovering block's own FIFO ring (its ring slice,
-//     slots reserved with LDS atomics) and are expanded by that block.  The
-//     only global queue is the deferred list: adjacency chunks of heavy
-//     vertices (> kHeavy entries) and of ring overflow.  Deferred chunks are
-//     processed in the next launch, statically strided over all blocks, so a
-//     hub is spread over the whole GPU and nothing ever spins.
+//   * Vertices are relabelled by total degree, descending, when the graph is
+//     uploaded, so the rows that take most of the random accesses (hubs)
+//     form one compact, cache-resident region; results are mapped back to
+//     the caller's ids at the end.
+//   * The grid is G fixed blocks.  Vertices are dealt to blocks in runs of
+//     kRun consecutive ids, round robin (run c goes to block c mod G): each
+//     run is one coalesced stretch of rows, and every block gets its share
+//     of hubs and of leaves, so the alive lists stay balanced at every level
+//     even though ids are sorted by degree.  Block b keeps its alive list
+//     compacted in place in its own list slice.
+//   * Vertices removed by a scan become adjacency chunks on the step's
+//     frontier list (one atomic per block and chunk of vertices); the
+//     cascade strides that list over all blocks.  Vertices removed during a
+//     cascade go to the discovering block's own FIFO ring (slots reserved
+//     with LDS atomics) and are expanded by that block; heavy ones
+//     (> kHeavy entries) and ring overflow go to the next step's list, so a
+//     hub is spread over the whole GPU.  Nothing ever spins.
+//   * Liveness is read from a per-instance bitmap (n bits: 100 MB for all
+//     eight instances of a 100M-vertex graph, so it stays in the 256 MB
+//     Infinity Cache) instead of the 4-byte index: a dead neighbour costs a
+//     cached bit, not a random HBM line.  The bit is cleared when the
+//     vertex is removed; the CAS on idx stays the authority.
 //   * One step = k_advance (1 block: per-instance state machine) + A (scan,
 //     or level-0 init) + B (cascade).  A step either starts a new level or
 //     continues the current one while deferred chunks remain.
@@ -49,6 +63,7 @@ constexpr int kWavesPerBlock = kBlock / kWave;
 constexpr int kChunk = 64;         // static chunk list granularity (densest pass)
 constexpr uint32_t kHeavy = 4096;  // vertices with more entries are deferred
 constexpr uint32_t kDefer = 4096;  // entries per deferred chunk
+constexpr uint32_t kRun = 64;      // consecutive vertex ids per ownership run
 
 enum Phase : int32_t { kPhaseScan = 0, kPhaseCont = 1, kPhaseDone = 2 };
 
@@ -56,19 +71,29 @@ enum Phase : int32_t { kPhaseScan = 0, kPhaseCont = 1, kPhaseDone = 2 };
 struct Inst {
     const int64_t* off;   // [n+1] merged adjacency offsets
     const uint32_t* adj;  // [D] entries u*L + l
+    const uint8_t* adjz;  // [D] z(u) of each entry's target (layers in which u has an edge)
+    uint32_t* alive;      // [ceil(n/32)] liveness bitmap
     uint32_t* deg;        // [n*L] residual layer degrees
     int32_t* cnt;         // [n]
     int32_t* idx;         // [n] core index, -1 while alive
     uint32_t* list;       // [G*cap] per-block alive lists
     uint32_t* ring;       // [G*cap] per-block removal rings
-    uint32_t* blk;        // [G*2] per block: alive list length, ring count from the scan
-    uint64_t* pl0;        // deferred chunk lists, by step parity
+    uint32_t* blk;        // [G*2] per block: alive list length, spare
+    uint64_t* pl0;        // frontier / deferred chunk lists, by step parity
     uint64_t* pl1;
     uint64_t plcap;
     uint32_t n, L, lmd, cap;
     uint32_t lmagic;  // min(floor(2^32 / L), 2^32 - 1)
     uint32_t flags;   // kFlagAggregate: combine decrements in LDS
 };
+
+__device__ __forceinline__ uint32_t* rec_of(const Inst& in, uint32_t v) { return in.deg + uint64_t(v) * in.L; }
+__device__ __forceinline__ int32_t* idx_of(const Inst& in, uint32_t v) { return in.idx + v; }
+__device__ __forceinline__ int32_t* cnt_of(const Inst& in, uint32_t v) { return in.cnt + v; }
+__device__ __forceinline__ bool is_alive(const Inst& in, uint32_t v) { return (in.alive[v >> 5] >> (v & 31)) & 1u; }
+__device__ __forceinline__ void clear_alive(const Inst& in, uint32_t v) {
+    atomicAnd(&in.alive[v >> 5], ~(1u << (v & 31)));
+}
 constexpr uint32_t kFlagAggregate = 1;
 
 // Per-instance state; the atomically updated words sit on lines of their own.
@@ -167,31 +192,77 @@ __device__ __forceinline__ void defer_push(const Inst& in, uint64_t* list, unsig
     }
 }
 
+// Block-aggregated append of the adjacency chunks of the threads' vertices
+// (defer == true): one atomic per block.  All threads call.
+__device__ __forceinline__ void block_defer_push(const Inst& in, uint64_t* list, unsigned long long* count,
+                                                 Control* ctl, bool defer, uint32_t v, uint32_t* s_tmp,
+                                                 unsigned long long* s_base) {
+    uint64_t beg = 0, dt = 0;
+    uint32_t nch = 0;
+    if (defer) {
+        beg = uint64_t(in.off[v]);
+        dt = uint64_t(in.off[v + 1]) - beg;
+        nch = uint32_t((dt + kDefer - 1) / kDefer);
+    }
+    uint32_t total;
+    const uint32_t ex = block_excl_scan(nch, &total, s_tmp);
+    if (total == 0) return;
+    if (threadIdx.x == 0) *s_base = atomicAdd(count, (unsigned long long)total);
+    __syncthreads();
+    const unsigned long long base = *s_base;
+    if (base + total > in.plcap) {  // cannot happen: plcap bounds all deferrals
+        if (threadIdx.x == 0) atomicCAS(&ctl->error, 0, int32_t(kErrDefer));
+        return;
+    }
+    for (uint32_t c = 0; c < nch; ++c) {
+        const uint64_t rem = dt - uint64_t(c) * kDefer;
+        list[base + ex + c] = make_item(beg + uint64_t(c) * kDefer, rem < kDefer ? uint32_t(rem) : kDefer);
+    }
+}
+
 // ----------------------------------------------------------- graph set-up
-// deg0[v][l] = deg_l(v) from per-layer offsets.
-__global__ void k_build_deg0(const int64_t* const* row_ptr, uint32_t n, uint32_t L, uint32_t* deg0) {
+// deg0[v][l] = deg_l(perm[v]) from per-layer offsets (v: internal id,
+// perm[v]: caller's id).
+__global__ void k_build_deg0(const int64_t* const* row_ptr, const uint32_t* perm, uint32_t n, uint32_t L,
+                             uint32_t* deg0) {
     const uint64_t total = uint64_t(n) * L;
     for (uint64_t t = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; t < total; t += uint64_t(gridDim.x) * blockDim.x) {
-        const uint32_t v = uint32_t(t / L), l = uint32_t(t % L);
+        const uint32_t v = perm[t / L], l = uint32_t(t % L);
         deg0[t] = uint32_t(row_ptr[l][v + 1] - row_ptr[l][v]);
     }
 }
 
-// Merged vertex-major adjacency; one wavefront per vertex.
+// Merged vertex-major adjacency in internal ids, and the target's z per
+// entry; one wavefront per vertex.
 __global__ void k_build_adj(const int64_t* const* row_ptr, const uint32_t* const* col, const uint32_t* deg0,
-                            const int64_t* off, uint32_t n, uint32_t L, uint32_t* adj) {
+                            const int64_t* off, const uint32_t* perm, const uint32_t* inv, const uint8_t* z_caller,
+                            uint32_t n, uint32_t L, uint32_t* adj, uint8_t* adjz) {
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
     const uint32_t nwaves = gridDim.x * blockDim.x / kWave;
     const uint32_t lane = lane_id();
     for (uint32_t v = wave; v < n; v += nwaves) {
+        const uint32_t ov = perm[v];
         uint64_t dst = uint64_t(off[v]);
         for (uint32_t l = 0; l < L; ++l) {
-            const int64_t s = row_ptr[l][v];
+            const int64_t s = row_ptr[l][ov];
             const uint32_t d = deg0[uint64_t(v) * L + l];
             const uint32_t* src = col[l] + s;
-            for (uint32_t i = lane; i < d; i += kWave) adj[dst + i] = src[i] * L + l;
+            for (uint32_t i = lane; i < d; i += kWave) {
+                adj[dst + i] = inv[src[i]] * L + l;
+                adjz[dst + i] = z_caller[src[i]];
+            }
             dst += d;
         }
+    }
+}
+
+// out[i][w] = core index of instance i at internal id inv[w]: results in
+// the caller's ids.
+__global__ void k_unpermute(const Inst* insts, const uint32_t* inv, uint32_t n, uint32_t I, int32_t* out) {
+    const uint64_t total = uint64_t(n) * I;
+    for (uint64_t t = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; t < total; t += uint64_t(gridDim.x) * blockDim.x) {
+        const uint64_t i = t / n, w = t % n;
+        out[t] = *idx_of(insts[i], inv[w]);
     }
 }
 
@@ -237,13 +308,15 @@ __global__ void k_advance(InstState* states, Control* ctl, uint32_t I, uint32_t 
 // ------------------------------------------------------------------ scans
 struct ScanShared {
     uint32_t keep;     // alive-list write position
-    uint32_t rem;      // ring write position
     uint32_t removed;  // removals by this block (for the alive counter)
+    uint32_t tmp[kWavesPerBlock + 1];
+    unsigned long long base;
 };
 
-// Ring append of removed light vertices / deferral of heavy ones (scan side:
-// the ring holds at most cap entries and the scan removes at most cap).
-__device__ __forceinline__ void scan_emit(const Inst& in, InstState* st, Control* ctl, ScanShared* sh, uint32_t* ring,
+// Survivors are appended to the block's alive list (in place: positions
+// never pass the chunk just read); removed vertices become chunks on the
+// frontier list of this step.  All threads call.
+__device__ __forceinline__ void scan_emit(const Inst& in, InstState* st, Control* ctl, ScanShared* sh,
                                           uint32_t parity, bool keep, bool rem, uint32_t v, uint32_t* list) {
     const uint64_t km = __ballot(keep);
     if (km) {
@@ -252,60 +325,153 @@ __device__ __forceinline__ void scan_emit(const Inst& in, InstState* st, Control
         base = __shfl(base, 0, kWave);
         if (keep) list[base + __builtin_amdgcn_mbcnt_hi(uint32_t(km >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(km), 0u))] = v;
     }
-    bool heavy = false, light = false;
-    if (rem) {
-        const uint64_t d = uint64_t(in.off[v + 1] - in.off[v]);
-        heavy = d > kHeavy;
-        light = d > 0 && !heavy;
-    }
     const uint64_t rm = __ballot(rem);
     if (rm && lane_id() == 0) atomicAdd(&sh->removed, uint32_t(__popcll(rm)));
-    const uint64_t lm = __ballot(light);
-    if (lm) {
-        uint32_t base = 0;
-        if (lane_id() == 0) base = atomicAdd(&sh->rem, uint32_t(__popcll(lm)));
-        base = __shfl(base, 0, kWave);
-        if (light) ring[base + __builtin_amdgcn_mbcnt_hi(uint32_t(lm >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(lm), 0u))] = v;
-    }
-    defer_push(in, defer_list(in, parity), &st->pcount[parity][0], ctl, heavy, v);
+    block_defer_push(in, defer_list(in, parity), &st->pcount[parity][0], ctl, rem, v, sh->tmp, &sh->base);
 }
 
-// Level 0 for every instance in one pass over deg0: a vertex survives
-// instance lmd iff it has >= lmd layers of degree >= 1.  Survivors get their
-// degree row, cnt and a place on the block's alive list; the others get
-// index 0 and go to the ring (or the deferred list).
+// ------------------------------------------------------------- level 0
+// Level 0 by pull (L <= kPullMaxL).  With z(v) = number of layers in which v
+// has an edge, every vertex with z(v) < lmd leaves instance lmd at once.
+// Instead of pushing their removal to all their neighbours, each vertex v
+// counts, per layer l and for every lmd <= z(v) at once,
+//     deg_lmd[v][l] = #{w in N_l(v) : z(w) >= lmd}
+// in one pass over its adjacency (16 register counters, no atomics).  The
+// level-0 scan then removes the survivors with fewer than lmd non-empty
+// layers; only those are pushed (their neighbours counted them).
+constexpr uint32_t kPullMaxL = 16;
+constexpr uint64_t kPullHeavy = 1024;  // vertices with more entries are pulled by a wavefront
+
+// c[j] += #{entries w of seg : z(w) > j}, j < kPullMaxL (segz = the
+// entries' target z, streamed sequentially).
+__device__ __forceinline__ void pull_count(const uint8_t* segz, uint32_t beg, uint32_t end, uint32_t step,
+                                           uint32_t* c) {
+    for (uint32_t t = beg; t < end; t += step) {
+        const uint32_t zw = segz[t];
+#pragma unroll
+        for (uint32_t j = 0; j < kPullMaxL; ++j) c[j] += zw > j;
+    }
+}
+
+// Hubs (ids < H, the heaviest vertices after the degree sort): one
+// wavefront per vertex, lanes strided over each layer segment.
+__global__ void __launch_bounds__(kBlock) k_pull_heavy(const Inst* insts, uint32_t I, const uint32_t* deg0,
+                                                       const uint8_t* z, uint32_t H, uint32_t L) {
+    const uint32_t lane = lane_id();
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const uint32_t nwaves = gridDim.x * blockDim.x / kWave;
+    const Inst& in0 = insts[0];
+    for (uint32_t v = wave; v < H; v += nwaves) {
+        const uint32_t zv = z[v];
+        uint64_t pos = uint64_t(in0.off[v]);
+        for (uint32_t l = 0; l < L; ++l) {
+            const uint32_t d = deg0[uint64_t(v) * L + l];
+            uint32_t c[kPullMaxL];
+#pragma unroll
+            for (uint32_t j = 0; j < kPullMaxL; ++j) c[j] = 0;
+            pull_count(in0.adjz + pos, lane, d, kWave, c);
+            pos += d;
+#pragma unroll
+            for (uint32_t j = 0; j < kPullMaxL; ++j)
+                for (int o = kWave / 2; o > 0; o >>= 1) c[j] += __shfl_xor(c[j], o, kWave);
+            if (lane == 0)
+                for (uint32_t i = 0; i < I && i < zv; ++i) insts[i].deg[uint64_t(v) * L + l] = c[i];
+        }
+    }
+}
+
+// Level 0 for every instance: degree rows (pulled, or copied from deg0 when
+// L > kPullMaxL), cnt, index, liveness bit and alive list; removed vertices
+// that neighbours still count go to the frontier list.  H: vertices below
+// it were pulled by k_pull_heavy.
 __global__ void __launch_bounds__(kBlock) k_init(const Inst* insts, InstState* states, Control* ctl, uint32_t I,
-                                                 const uint32_t* deg0, uint32_t n, uint32_t L, uint32_t cap) {
+                                                 const uint32_t* deg0, const uint8_t* z, uint32_t n, uint32_t L,
+                                                 uint32_t cap, uint32_t H, uint32_t pull) {
     __shared__ ScanShared sh[64];
-    for (uint32_t i = threadIdx.x; i < I; i += blockDim.x) sh[i] = ScanShared{0u, 0u, 0u};
+    for (uint32_t i = threadIdx.x; i < I; i += blockDim.x) {
+        sh[i].keep = 0;
+        sh[i].removed = 0;
+    }
     __syncthreads();
-    const uint64_t lo = uint64_t(blockIdx.x) * cap;  // this block's list / ring slice
+    const uint64_t lo = uint64_t(blockIdx.x) * cap;  // this block's list slice
     const uint32_t G = gridDim.x;
-    const uint32_t own = blockIdx.x < n ? uint32_t((uint64_t(n) - blockIdx.x + G - 1) / G) : 0u;  // b, b+G, ...
-    for (uint32_t base = 0; base < own; base += kBlock) {
-        const uint32_t t = base + threadIdx.x;
-        const bool valid = t < own;
-        const uint32_t v = valid ? blockIdx.x + t * G : 0u;
-        uint32_t nz = 0;
-        if (valid)
-            for (uint32_t l = 0; l < L; ++l) nz += deg0[uint64_t(v) * L + l] >= 1u;
-        if (valid) {
-            // instances lmd <= nz keep v: copy the row once per such instance
-            for (uint32_t l = 0; l < L; ++l) {
-                const uint32_t d = deg0[uint64_t(v) * L + l];
-                for (uint32_t i = 0; i < I && insts[i].lmd <= nz; ++i) insts[i].deg[uint64_t(v) * L + l] = d;
+    // runs owned: c = b, b+G, ... < ceil(n / kRun); cap >= their vertex count
+    const uint64_t nruns = (uint64_t(n) + kRun - 1) / kRun;
+    const uint32_t myruns = blockIdx.x < nruns ? uint32_t((nruns - blockIdx.x + G - 1) / G) : 0u;
+    const Inst& in0 = insts[0];
+    for (uint32_t base = 0; base < myruns * kRun; base += kBlock) {
+        const uint32_t m = base + threadIdx.x;  // m-th candidate of this block
+        const uint64_t vv = (uint64_t(m / kRun) * G + blockIdx.x) * kRun + m % kRun;
+        const bool valid = m < myruns * kRun && vv < n;
+        const uint32_t v = valid ? uint32_t(vv) : 0u;
+        const uint32_t zv = valid ? uint32_t(z[v]) : 0u;
+        if (valid && zv > 0) {
+            if (!pull) {
+                // copy the row once per instance lmd <= z(v)
+                for (uint32_t l = 0; l < L; ++l) {
+                    const uint32_t d = deg0[uint64_t(v) * L + l];
+                    for (uint32_t i = 0; i < I && i < zv; ++i) rec_of(insts[i], v)[l] = d;
+                }
+            } else if (v >= H) {
+                uint64_t pos = uint64_t(in0.off[v]);
+                for (uint32_t l = 0; l < L; ++l) {
+                    const uint32_t d = deg0[uint64_t(v) * L + l];
+                    uint32_t c[kPullMaxL];
+#pragma unroll
+                    for (uint32_t j = 0; j < kPullMaxL; ++j) c[j] = 0;
+                    pull_count(in0.adjz + pos, 0, d, 1, c);
+                    pos += d;
+#pragma unroll
+                    for (uint32_t j = 0; j < kPullMaxL; ++j)
+                        if (j < I && j < zv) insts[j].deg[uint64_t(v) * L + l] = c[j];
+                }
             }
         }
         for (uint32_t i = 0; i < I; ++i) {
-            if (states[i].phase == kPhaseDone) continue;  // lmd above every vertex's layer count: all zero
             const Inst& in = insts[i];
-            const bool alive = valid && nz >= in.lmd;
-            const bool dead = valid && !alive;
-            if (valid) {
-                in.idx[v] = alive ? -1 : 0;
-                if (alive) in.cnt[v] = int32_t(nz);
+            if (states[i].phase == kPhaseDone) {  // lmd above every vertex's layer count: all zero
+                if (valid) *idx_of(in, v) = 0;
+                continue;
             }
-            scan_emit(in, &states[i], ctl, &sh[i], in.ring + lo, 0u, alive, dead, v, in.list + lo);
+            static_assert(kRun == kWave, "bitmap words are written per wavefront run");
+            // layers with degree >= 1 among the rows just built
+            uint32_t c1 = 0;
+            if (valid && zv >= in.lmd) {
+                if (pull) {
+                    const uint32_t* d = rec_of(in, v);
+                    for (uint32_t l = 0; l < L; ++l) c1 += d[l] >= 1u;
+                } else {
+                    c1 = zv;
+                }
+            }
+            const bool counted = valid && zv >= in.lmd;  // neighbours' rows include v
+            const bool alive = counted && c1 >= in.lmd;
+            if (valid) {
+                *idx_of(in, v) = alive ? -1 : 0;
+                if (alive) *cnt_of(in, v) = int32_t(c1);
+            }
+            {  // this wavefront's lanes are the 64 ids of one aligned run
+                const uint64_t am = __ballot(alive);
+                const uint64_t run0 = (uint64_t((base + threadIdx.x) / kRun) * G + blockIdx.x) * kRun;
+                if (lane_id() == 0 && run0 < n) {
+                    in.alive[run0 >> 5] = uint32_t(am);
+                    if (run0 + 32 < n) in.alive[(run0 >> 5) + 1] = uint32_t(am >> 32);
+                }
+            }
+            // pull: only vertices that neighbours counted are pushed; push:
+            // every removed vertex is
+            const bool push = pull ? (counted && !alive) : (valid && !alive);
+            const uint64_t rm = __ballot(valid && !alive);
+            if (rm && lane_id() == 0) atomicAdd(&sh[i].removed, uint32_t(__popcll(rm)));
+            const uint64_t km = __ballot(alive);
+            if (km) {
+                uint32_t kb = 0;
+                if (lane_id() == 0) kb = atomicAdd(&sh[i].keep, uint32_t(__popcll(km)));
+                kb = __shfl(kb, 0, kWave);
+                if (alive)
+                    in.list[lo + kb + __builtin_amdgcn_mbcnt_hi(uint32_t(km >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(km), 0u))] = v;
+            }
+            block_defer_push(in, defer_list(in, 0u), &states[i].pcount[0][0], ctl, push, v, sh[i].tmp, &sh[i].base);
         }
     }
     __syncthreads();
@@ -313,7 +479,6 @@ __global__ void __launch_bounds__(kBlock) k_init(const Inst* insts, InstState* s
         const Inst& in = insts[i];
         if (states[i].phase == kPhaseDone) continue;
         in.blk[2 * blockIdx.x] = sh[i].keep;
-        in.blk[2 * blockIdx.x + 1] = sh[i].rem;
         if (sh[i].removed) atomicAdd(&states[i].alive[0], 0ull - (unsigned long long)sh[i].removed);
     }
 }
@@ -335,7 +500,10 @@ __global__ void __launch_bounds__(kBlock) k_scan(const Inst* insts, InstState* s
         const uint64_t lo = uint64_t(blockIdx.x) * in.cap;
         uint32_t* list = in.list + lo;
         const uint32_t len = in.blk[2 * blockIdx.x];
-        if (threadIdx.x == 0) sh = ScanShared{0u, 0u, 0u};
+        if (threadIdx.x == 0) {
+            sh.keep = 0;
+            sh.removed = 0;
+        }
         __syncthreads();
         for (uint32_t base = 0; base < len; base += kBlock) {
             const uint32_t t = base + threadIdx.x;
@@ -343,24 +511,26 @@ __global__ void __launch_bounds__(kBlock) k_scan(const Inst* insts, InstState* s
             bool live = false;
             if (t < len) {
                 v = list[t];
-                live = in.idx[v] < 0;
+                live = is_alive(in, v);
             }
             uint32_t c = 0;
             if (live) {
-                const uint32_t* d = in.deg + uint64_t(v) * L;
+                const uint32_t* d = rec_of(in, v);
                 for (uint32_t l = 0; l < L; ++l) c += d[l] >= thr;
             }
             const bool rem = live && c < in.lmd;
             const bool keep = live && !rem;
-            if (rem) in.idx[v] = k;
-            if (keep) in.cnt[v] = int32_t(c);
+            if (rem) {
+                *idx_of(in, v) = k;
+                clear_alive(in, v);
+            }
+            if (keep) *cnt_of(in, v) = int32_t(c);
             __syncthreads();  // every entry of this chunk is read before compaction writes
-            scan_emit(in, st, ctl, &sh, in.ring + lo, s & 1u, keep, rem, v, list);
+            scan_emit(in, st, ctl, &sh, s & 1u, keep, rem, v, list);
         }
         __syncthreads();
         if (threadIdx.x == 0) {
             in.blk[2 * blockIdx.x] = sh.keep;
-            in.blk[2 * blockIdx.x + 1] = sh.rem;
             if (sh.removed) atomicAdd(&st->alive[0], 0ull - (unsigned long long)sh.removed);
         }
         __syncthreads();
@@ -433,16 +603,28 @@ __device__ __forceinline__ void emit_removed(const Inst& in, InstState* st, Cont
     defer_push(in, defer_list(in, next_parity), &st->pcount[next_parity][0], ctl, rem && !placed, u);
 }
 
-// Applies c decrements to deg[e] (u = e / L) and the cascade rule; returns
-// true when this call removed u.
-__device__ __forceinline__ bool apply_dec(const Inst& in, uint32_t e, uint32_t u, uint32_t c, uint32_t thr,
-                                          int32_t k) {
-    const uint32_t old = atomicSub(&in.deg[e], c);
-    if (old >= thr && old - c < thr) {
-        const int32_t oc = atomicSub(&in.cnt[u], 1);
-        if (oc == int32_t(in.lmd)) return atomicCAS(&in.idx[u], -1, k) == -1;
-    }
-    return false;
+// Applies U decrements deg[e[r]] -= c[r] (where valid[r]) with the cascade
+// rule; rem[r] = this call removed u[r].  The U subtractions are issued
+// together, then the (rare) cnt updates, then the CAS: three dependent
+// round trips per batch instead of three per entry.
+template <uint32_t U>
+__device__ __forceinline__ void apply_batch(const Inst& in, const uint32_t* e, const uint32_t* u, const uint32_t* c,
+                                            const bool* valid, uint32_t thr, int32_t k, bool* rem) {
+    uint32_t old[U];
+#pragma unroll
+    for (uint32_t r = 0; r < U; ++r) old[r] = valid[r] ? atomicSub(&in.deg[e[r]], c[r]) : 0u;
+    bool cross[U];
+#pragma unroll
+    for (uint32_t r = 0; r < U; ++r) cross[r] = valid[r] && old[r] >= thr && old[r] - c[r] < thr;
+    int32_t oc[U];
+#pragma unroll
+    for (uint32_t r = 0; r < U; ++r) oc[r] = cross[r] ? atomicSub(&in.cnt[u[r]], 1) : 0;
+#pragma unroll
+    for (uint32_t r = 0; r < U; ++r)
+        rem[r] = cross[r] && oc[r] == int32_t(in.lmd) && atomicCAS(&in.idx[u[r]], -1, k) == -1;
+#pragma unroll
+    for (uint32_t r = 0; r < U; ++r)
+        if (rem[r]) clear_alive(in, u[r]);
 }
 
 // Applies every combined decrement in the LDS table and empties it.  All
@@ -451,30 +633,61 @@ __device__ __forceinline__ void agg_flush(const Inst& in, InstState* st, Control
                                           uint32_t next_parity, uint32_t head, uint32_t thr, int32_t k) {
     __syncthreads();
     const uint32_t used = sh->used;
-    for (uint32_t t0 = 0; t0 < used; t0 += kBlock) {
-        const uint32_t t = t0 + threadIdx.x;
-        bool rem = false;
-        uint32_t u = 0;
-        if (t < used) {
-            const uint32_t slot = sh->slots[t];
-            const uint32_t e = sh->key[slot];
-            const uint32_t c = sh->cnt[slot];
-            sh->key[slot] = kAggEmpty;
-            sh->cnt[slot] = 0;
-            u = div_L(e, in.L, in.lmagic);
-            rem = apply_dec(in, e, u, c, thr, k);
+    constexpr uint32_t U = 4;
+    for (uint32_t t0 = 0; t0 < used; t0 += kBlock * U) {
+        uint32_t e[U], u[U], c[U];
+        bool valid[U], rem[U];
+#pragma unroll
+        for (uint32_t r = 0; r < U; ++r) {
+            const uint32_t t = t0 + r * kBlock + threadIdx.x;
+            valid[r] = t < used;
+            e[r] = 0;
+            c[r] = 0;
+            if (valid[r]) {
+                const uint32_t slot = sh->slots[t];
+                e[r] = sh->key[slot];
+                c[r] = sh->cnt[slot];
+                sh->key[slot] = kAggEmpty;
+                sh->cnt[slot] = 0;
+            }
+            u[r] = div_L(e[r], in.L, in.lmagic);
         }
-        emit_removed(in, st, ctl, sh, ring, next_parity, head, rem, u);
+        apply_batch<U>(in, e, u, c, valid, thr, k, rem);
+#pragma unroll
+        for (uint32_t r = 0; r < U; ++r) emit_removed(in, st, ctl, sh, ring, next_parity, head, rem[r], u[r]);
     }
     __syncthreads();
     if (threadIdx.x == 0) sh->used = 0;
     __syncthreads();
 }
 
+// Inserts one decrement of deg[e] into the LDS table; false if the probe
+// sequence is full (the caller then applies it directly).
+__device__ __forceinline__ bool agg_insert(CascShared* sh, uint32_t e) {
+    uint32_t h = (e * 2654435761u) >> 20;  // 12-bit hash
+    for (uint32_t p = 0; p < kAggProbes; ++p, h = (h + 1) & (kAggSlots - 1)) {
+        uint32_t kk = sh->key[h];
+        if (kk == kAggEmpty) {
+            kk = atomicCAS(&sh->key[h], kAggEmpty, e);
+            if (kk == kAggEmpty) {
+                sh->slots[atomicAdd(&sh->used, 1u)] = uint16_t(h);
+                kk = e;
+            }
+        }
+        if (kk == e) {
+            atomicAdd(&sh->cnt[h], 1u);
+            return true;
+        }
+    }
+    return false;
+}
+
 // Expands the adjacency ranges (start, len) held by the block's threads
 // (len = 0 for none) and applies the cascade rule to every entry whose
-// target is alive.  All threads call; the LDS table must be empty on entry
-// and is empty on return.
+// target is alive.  Each thread walks kUnroll entries per round with their
+// loads issued together (the walk is latency-bound).  All threads call; the
+// LDS table must be empty on entry and is empty on return.
+constexpr uint32_t kUnroll = 4;
 __device__ __forceinline__ void cascade_expand(const Inst& in, InstState* st, Control* ctl, CascShared* sh,
                                                uint32_t* ring, uint32_t next_parity, int32_t k, uint64_t start,
                                                uint32_t len) {
@@ -483,53 +696,48 @@ __device__ __forceinline__ void cascade_expand(const Inst& in, InstState* st, Co
     sh->pre[threadIdx.x] = pre;
     sh->start[threadIdx.x] = start;
     __syncthreads();
-    const uint32_t L = in.L, magic = in.lmagic;
+    const uint32_t L = in.L, magic = in.lmagic, lmd = in.lmd;
     const uint32_t thr = uint32_t(k) + 1u;
     const uint32_t head = sh->head;
     const bool agg = (in.flags & kFlagAggregate) != 0;
-    for (uint32_t j0 = 0; j0 < total; j0 += kBlock) {
-        const uint32_t j = j0 + threadIdx.x;
-        bool rem = false;
-        uint32_t u = 0;
-        if (j < total) {
-            uint32_t lo = 0, hi = kBlock - 1;  // largest p with pre[p] <= j
-            while (lo < hi) {
-                const uint32_t mid = (lo + hi + 1) >> 1;
-                if (sh->pre[mid] <= j) lo = mid;
-                else hi = mid - 1;
-            }
-            const uint32_t e = in.adj[sh->start[lo] + (j - sh->pre[lo])];
-            u = div_L(e, L, magic);
-            if (in.idx[u] < 0) {
-                bool done = false;
-                if (agg) {  // combine in the LDS table; fall back to a direct atomic
-                    uint32_t h = (e * 2654435761u) >> 20;  // 12-bit hash
-                    for (uint32_t p = 0; p < kAggProbes && !done; ++p, h = (h + 1) & (kAggSlots - 1)) {
-                        uint32_t kk = sh->key[h];
-                        if (kk == kAggEmpty) {
-                            kk = atomicCAS(&sh->key[h], kAggEmpty, e);
-                            if (kk == kAggEmpty) {
-                                sh->slots[atomicAdd(&sh->used, 1u)] = uint16_t(h);
-                                kk = e;
-                            }
-                        }
-                        if (kk == e) {
-                            atomicAdd(&sh->cnt[h], 1u);
-                            done = true;
-                        }
-                    }
+    for (uint32_t j0 = 0; j0 < total; j0 += kBlock * kUnroll) {
+        uint32_t e[kUnroll], u[kUnroll];
+        bool live[kUnroll];
+#pragma unroll
+        for (uint32_t r = 0; r < kUnroll; ++r) {
+            const uint32_t j = j0 + r * kBlock + threadIdx.x;
+            live[r] = j < total;
+            e[r] = 0;
+            if (live[r]) {
+                uint32_t lo = 0, hi = kBlock - 1;  // largest p with pre[p] <= j
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi + 1) >> 1;
+                    if (sh->pre[mid] <= j) lo = mid;
+                    else hi = mid - 1;
                 }
-                if (!done) rem = apply_dec(in, e, u, 1u, thr, k);
+                const uint64_t p = sh->start[lo] + (j - sh->pre[lo]);
+                e[r] = in.adj[p];
+                live[r] = in.adjz[p] >= lmd;  // z(target) < lmd: never alive in this instance
             }
         }
-        emit_removed(in, st, ctl, sh, ring, next_parity, head, rem, u);
-        if (agg) {
-            __syncthreads();
-            if (sh->used > kAggSlots / 2) agg_flush(in, st, ctl, sh, ring, next_parity, head, thr, k);
+#pragma unroll
+        for (uint32_t r = 0; r < kUnroll; ++r) {
+            u[r] = div_L(e[r], L, magic);
+            if (live[r]) live[r] = is_alive(in, u[r]);
         }
+        bool rem[kUnroll], direct[kUnroll];
+        uint32_t one[kUnroll];
+#pragma unroll
+        for (uint32_t r = 0; r < kUnroll; ++r) {
+            one[r] = 1u;
+            direct[r] = live[r] && !(agg && agg_insert(sh, e[r]));
+        }
+        apply_batch<kUnroll>(in, e, u, one, direct, thr, k, rem);
+#pragma unroll
+        for (uint32_t r = 0; r < kUnroll; ++r) emit_removed(in, st, ctl, sh, ring, next_parity, head, rem[r], u[r]);
+        if (agg) agg_flush(in, st, ctl, sh, ring, next_parity, head, thr, k);
     }
-    if (agg) agg_flush(in, st, ctl, sh, ring, next_parity, head, thr, k);
-    else __syncthreads();
+    __syncthreads();
 }
 
 // Cascade of every instance not done: deferred chunks of this step's list
@@ -547,18 +755,19 @@ __global__ void __launch_bounds__(kBlock) k_cascade(const Inst* insts, InstState
         InstState* st = &states[i];
         const int32_t phase = st->phase;
         if (phase == kPhaseDone) continue;
+        const uint32_t par = s & 1u, npar = par ^ 1u;
+        const uint64_t pc = st->pcount[par][0];
+        if (pc == 0) continue;  // empty frontier: the ring starts empty, nothing to do
         const Inst& in = insts[i];
         const int32_t k = st->k;
-        const uint32_t par = s & 1u, npar = par ^ 1u;
         uint32_t* ring = in.ring + uint64_t(b) * in.cap;
         if (threadIdx.x == 0) {
             sh.head = 0;
-            sh.tail = phase == kPhaseScan ? in.blk[2 * b + 1] : 0u;
+            sh.tail = 0;
             sh.removed = 0;
         }
         __syncthreads();
-        // deferred chunks
-        const uint64_t pc = st->pcount[par][0];
+        // frontier / deferred chunks
         const uint64_t* pl = defer_list(in, par);
         for (uint64_t m0 = 0; m0 * G + b < pc; m0 += kBlock) {
             const uint64_t j = (m0 + threadIdx.x) * G + b;
@@ -588,7 +797,6 @@ __global__ void __launch_bounds__(kBlock) k_cascade(const Inst* insts, InstState
             __syncthreads();
         }
         if (threadIdx.x == 0) {
-            in.blk[2 * b + 1] = 0;
             if (sh.removed) atomicAdd(&st->alive[0], 0ull - (unsigned long long)sh.removed);
         }
         __syncthreads();
@@ -654,7 +862,7 @@ __global__ void __launch_bounds__(kHistBlock) k_edge_hist(const Inst in, const u
             s = b + ci * kChunk;
             len = uint32_t((e - s) < uint64_t(kChunk) ? (e - s) : uint64_t(kChunk));
         }
-        const int32_t iv = len ? in.idx[v] : 0;
+        const int32_t iv = len ? *idx_of(in, v) : 0;
         if (iv < 1) len = 0;  // v outside S(1): contributes to no cell
         uint32_t total;
         const uint32_t pre = wave_excl_scan(len, &total);
@@ -674,7 +882,7 @@ __global__ void __launch_bounds__(kHistBlock) k_edge_hist(const Inst in, const u
                 const uint32_t e = in.adj[s_start[lo] + (j - s_pre[lo])];
                 const uint32_t u = div_L(e, L, in.lmagic);
                 const uint32_t l = e - u * L;
-                const int32_t iu = in.idx[u];
+                const int32_t iu = *idx_of(in, u);
                 const uint32_t m = uint32_t(iu < int32_t(s_src[lo]) ? iu : int32_t(s_src[lo]));
                 if (iu >= 1) {
                     if (m < lds_k) atomicAdd(&s_hist[l * lds_k + m], 1u);
