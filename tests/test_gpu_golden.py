@@ -1,0 +1,135 @@
+"""GPU tests against the reference's own outputs (tests/golden/) and
+size-independent properties at a full benchmark size."""
+import dataclasses
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from pargridcpu_amd.graph import C2, MultilayerGraph
+from pargridcpu_amd.hipgrid import Engine, HipGrid
+from tests.golden.make_golden import GRAPHS, LOADER_LAYERS, graph_checksum
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def fixture(name):
+    return dict(np.load(os.path.join(GOLDEN, name + ".npz")))
+
+
+@pytest.mark.parametrize("name", sorted(GRAPHS))
+def test_engine_matches_reference(name):
+    fx = fixture(name)
+    g = MultilayerGraph.generate(GRAPHS[name])
+    assert graph_checksum(g) == str(fx["checksum"])
+    say(f"case golden {name} n={g.n} L={g.L}")
+    with Engine.from_graph(g) as eng:
+        eng.decompose()
+        ci = eng.core_index()
+        cl, kmax = eng.cells()
+        dens = {}
+        for beta in (1.0, 0.5, 2.0):
+            w = eng.densest(beta)
+            dens[beta] = (np.float32(w.max_density), [w.res_k, w.res_lmd, w.res_len], eng.densest_vertices())
+    assert np.array_equal(ci, fx["core_index"])
+    rows = sorted((i + 1, k, int(cl[i, k])) for i in range(g.L) for k in range(1, int(kmax[i]) + 1))
+    assert np.array_equal(np.array(rows, dtype=np.int64).reshape(-1, 3), fx["cells"])
+    assert int(kmax.sum()) == int(fx["lpe_count"])
+    for beta, (md, res, verts) in dens.items():
+        tag = str(beta).replace(".", "p")
+        assert md == fx[f"wds_{tag}"][0]
+        assert res == fx[f"wds_{tag}_res"].tolist()
+        assert np.array_equal(verts, np.nonzero(fx["core_index"][res[1] - 1] >= res[0])[0].astype(np.uint32))
+
+
+def test_loader_case(tmp_path):
+    fx = fixture("loader")
+    d = tmp_path / "loader"
+    d.mkdir()
+    (d / "mlg.conf").write_text("".join(f"l{i}.txt\n" for i in range(len(LOADER_LAYERS))))
+    for i, text in enumerate(LOADER_LAYERS):
+        (d / f"l{i}.txt").write_text(text)
+    g = MultilayerGraph.LoadFromFile(str(d) + "/")
+    say("case loader")
+    tree = HipGrid.Execute(g)
+    assert np.array_equal(tree.core_index, fx["core_index"])
+    assert tree.getCoreByKAndLmd(2, 1).tolist() == list(range(6))
+    assert tree.getCoreByKAndLmd(3, 1) is None
+
+
+def test_driver_wds_c1(tmp_path):
+    fx = fixture("c1")
+    g = MultilayerGraph.generate(GRAPHS["c1"])
+    root = tmp_path / "dataset"
+    g.write_text(str(root / "c1"))
+    exe = os.path.join(ROOT, "pargridcpu_amd", "bin", "hipgrid")
+    say("case driver HipGridWds c1")
+    r = subprocess.run([exe, "-d", "c1", "-m", "HipGridWds", "-o", "1", "-root", str(root)], capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    out = r.stdout
+    assert "|L| = 7, |V| = 20000" in out
+    assert re.search(r"count = (\d+)", out).group(1) == str(int(fx["lpe_count"]))
+    assert re.search(r"res_k = (\d+)", out).group(1) == str(fx["wds_1p0_res"][0])
+    assert re.search(r"res_lmd = (\d+)", out).group(1) == str(fx["wds_1p0_res"][1])
+    assert re.search(r"res_len = (\d+)", out).group(1) == str(fx["wds_1p0_res"][2])
+    md = float(re.search(r"maximum_density = ([0-9.e+-]+)", out).group(1))
+    assert abs(md - float(fx["wds_1p0"][0])) <= 1e-4 * float(fx["wds_1p0"][0])  # printed with %g
+    assert "mem = " in out and "HipGrid Elapsed time:" in out
+
+
+def test_int32_offsets_and_layer_order():
+    g = MultilayerGraph.generate(GRAPHS["g2000_L5"])
+    fx = fixture("g2000_L5")
+    say("case int32 offsets")
+    with Engine(g.layers(), g.n, offset_bits=32) as eng:
+        eng.decompose()
+        assert np.array_equal(eng.core_index(), fx["core_index"])
+    say("case layer orders")
+    for o in (1, 2, 3, 4):
+        g.SetGraphOrder(o)
+        with Engine.from_graph(g) as eng:
+            eng.decompose()
+            assert np.array_equal(eng.core_index(), fx["core_index"])
+
+
+def test_full_size_c2_properties():
+    """C2 at full size (n = 1M, L = 10): bit-exact against the oracle's
+    CoreIndex restatement, plus structural invariants and run-to-run
+    determinism."""
+    g = MultilayerGraph.generate(C2)
+    say(f"case C2 full size n={g.n} D={g.directed_entries()}")
+    with Engine.from_graph(g) as eng:
+        eng.decompose()
+        a = eng.core_index()
+        cl, kmax = eng.cells()
+        off, vtx, kcap = eng.shells()
+        eng.decompose()
+        b = eng.core_index()
+    assert np.array_equal(a, b)
+    assert (a[1:] <= a[:-1]).all()                      # nesting in lmd
+    assert (kmax[1:] <= kmax[:-1]).all()                # staircase
+    assert (a >= 0).all()
+    for i in range(g.L):                                # cell lengths = suffix counts
+        h = np.bincount(a[i], minlength=int(kmax[i]) + 1)
+        assert np.array_equal(cl[i, :kmax[i] + 1], np.cumsum(h[::-1])[::-1])
+    stride = kcap + 1
+    assert off[-1] == g.n * g.L                         # shells partition every (lmd, v)
+    say("case C2 oracle (CPU, ~10 s)")
+    ref = O.coreindex(g.n, g.layers())
+    assert np.array_equal(a, ref)
+    i = 0
+    for k in (0, 1, int(kmax[0])):
+        s = vtx[off[i * stride + k]:off[i * stride + k + 1]]
+        assert np.array_equal(s, np.nonzero(ref[i] == k)[0].astype(np.uint32))
