@@ -78,7 +78,7 @@ def cpu_baseline(spec, kind: str, budget_s: float = 30.0):
                 raise RuntimeError("reference ParGrid printed no elapsed time:\n" + out.stdout[-2000:] + out.stderr)
             desc += (f"; reference ParGrid (main.cpp -m ParGrid, -O3), -num_thread {threads}, peel team fixed at 32 "
                      f"threads (FCSyncLeft.cpp:51), timed by its own omp_get_wtime bracket")
-            cores = min(32, os.cpu_count() or 1)
+            cores = 32  # the peel team is fixed at 32 threads whatever -num_thread says
         finally:
             shutil.rmtree(tmp, ignore_errors=True)
     else:
@@ -99,7 +99,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default=os.environ.get("HG_WORKLOAD", "c2"))
+    ap.add_argument("--workload", default=os.environ.get("HG_WORKLOAD", "c3"))
     ap.add_argument("--cpu-baseline", default="auto", choices=["auto", "reference", "port", "off"])
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (counter calibration)")
