@@ -41,6 +41,19 @@ def algorithmic_bytes(n: int, L: int, D: int) -> int:
     return L * (12 * D + 8 * L * (n + 1) + 4 * n * L + 8 * n)
 
 
+def measured_traffic(workload: str):
+    """HBM bytes per decomposition from the committed rocprofv3 counter
+    summary of this workload (FETCH_SIZE and WRITE_SIZE runs, calibrated
+    against a 1 GiB copy), or None when no summary exists."""
+    path = os.path.join(ROOT, "profiles", f"hg_decompose_{workload}.md")
+    if not os.path.exists(path):
+        return None, None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_decomposition:"):
+            return int(line.split(":")[1]), os.path.relpath(path, ROOT)
+    return None, None
+
+
 def cpu_baseline(spec, kind: str, budget_s: float = 30.0):
     """Times the reference's ParGrid (built from source into oracle/_ref,
     -O3) or the oracle's single-thread ParGrid restatement on a bounded
@@ -156,8 +169,9 @@ def main() -> int:
     balg = algorithmic_bytes(n, L, D)
     dev_ms = sum(per_call) / len(per_call)
     achieved = balg / (dev_ms * 1e-3) / 1e9
+    traffic, traffic_src = measured_traffic(args.workload)
     roof = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic, "traffic_source": traffic_src,
             "kernel": "hg_decompose (k_init + per-level k_scan/k_cascade)", "algorithmic_bytes": balg,
             "avg_ms": round(dev_ms, 4)}
     cpu = None
