@@ -30,43 +30,13 @@
 #include <vector>
 
 #include "hipgrid.h"
+#include "hipgrid_common.h"
 #include "hipgrid_kernels.h"
 
 using namespace hg;
-
-namespace {
-
-thread_local std::string g_err;
-
-int fail(int code, const std::string& msg) {
-    g_err = msg;
-    return code;
-}
-
-#define HIP_TRY(expr)                                                                                  \
-    do {                                                                                               \
-        hipError_t _e = (expr);                                                                        \
-        if (_e != hipSuccess) {                                                                        \
-            return fail(_e == hipErrorOutOfMemory ? HG_ERR_NOMEM : HG_ERR_HIP,                         \
-                        std::string(#expr) + ": " + hipGetErrorString(_e));                           \
-        }                                                                                              \
-    } while (0)
-
-int env_int(const char* name, int dflt) {
-    const char* v = std::getenv(name);
-    if (!v || !*v) return dflt;
-    const int x = std::atoi(v);
-    return x > 0 ? x : dflt;
-}
-
-template <typename T>
-hipError_t dalloc(T** p, size_t count) {
-    *p = nullptr;
-    if (count == 0) count = 1;
-    return hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T));
-}
-
-}  // namespace
+using hgc::dalloc;
+using hgc::env_int;
+using hgc::fail;
 
 struct hg_graph {
     int device = 0;
@@ -564,7 +534,7 @@ int densest_impl(hg_graph* g, double beta, hg_wds* out) {
 
 extern "C" {
 
-const char* hg_last_error(void) { return g_err.c_str(); }
+const char* hg_last_error(void) { return hgc::g_err.c_str(); }
 
 int hg_graph_create(const hg_layer_csr* layers, int32_t L, int64_t n, int32_t offset_bits, void* stream,
                     hg_graph** out) {

@@ -16,7 +16,9 @@
 //   cascade  each entry of a removed vertex decrements deg[u][l] with an
 //            atomic; the unique decrement taking it from k+1 to k decrements
 //            cnt[u]; the unique decrement taking cnt[u] from lmd to lmd-1
-//            removes u (CAS on idx).
+//            removes u.  FCSyncLeft guards the removal with a CAS on valid[u];
+//            here every vertex that is not alive holds cnt < lmd, which makes
+//            that decrement unique over the whole run, so no CAS is needed.
 //
 // Scheduling, designed around the cost of same-address device atomics
 // (they serialise, ~11 ns each on MI355X, while scattered atomics reach
@@ -42,7 +44,8 @@
 //     eight instances of a 100M-vertex graph, so it stays in the 256 MB
 //     Infinity Cache) instead of the 4-byte index: a dead neighbour costs a
 //     cached bit, not a random HBM line.  The bit is cleared when the
-//     vertex is removed; the CAS on idx stays the authority.
+//     vertex is removed; a stale bit only costs a harmless decrement of a
+//     removed vertex (its cnt is already below lmd).
 //   * One step = k_advance (1 block: per-instance state machine) + A (scan,
 //     or level-0 init) + B (cascade).  A step either starts a new level or
 //     continues the current one while deferred chunks remain.
@@ -448,7 +451,7 @@ __global__ void __launch_bounds__(kBlock) k_init(const Inst* insts, InstState* s
             const bool alive = counted && c1 >= in.lmd;
             if (valid) {
                 *idx_of(in, v) = alive ? -1 : 0;
-                if (alive) *cnt_of(in, v) = int32_t(c1);
+                *cnt_of(in, v) = int32_t(c1);  // < lmd for every vertex not alive
             }
             {  // this wavefront's lanes are the 64 ids of one aligned run
                 const uint64_t am = __ballot(alive);
@@ -524,7 +527,7 @@ __global__ void __launch_bounds__(kBlock) k_scan(const Inst* insts, InstState* s
                 *idx_of(in, v) = k;
                 clear_alive(in, v);
             }
-            if (keep) *cnt_of(in, v) = int32_t(c);
+            if (live) *cnt_of(in, v) = int32_t(c);
             __syncthreads();  // every entry of this chunk is read before compaction writes
             scan_emit(in, st, ctl, &sh, s & 1u, keep, rem, v, list);
         }
@@ -605,8 +608,12 @@ __device__ __forceinline__ void emit_removed(const Inst& in, InstState* st, Cont
 
 // Applies U decrements deg[e[r]] -= c[r] (where valid[r]) with the cascade
 // rule; rem[r] = this call removed u[r].  The U subtractions are issued
-// together, then the (rare) cnt updates, then the CAS: three dependent
-// round trips per batch instead of three per entry.
+// together, then the (rare) cnt updates: two dependent round trips per batch
+// instead of two per entry.  The cnt subtraction that returns lmd is unique
+// over the whole decomposition, so it alone decides the removal (no CAS on
+// idx): every vertex that is not alive holds cnt < lmd (the scans and level
+// 0 store cnt for the vertices they remove, a cascade removal leaves
+// lmd - 1), and cnt only decreases between scans.
 template <uint32_t U>
 __device__ __forceinline__ void apply_batch(const Inst& in, const uint32_t* e, const uint32_t* u, const uint32_t* c,
                                             const bool* valid, uint32_t thr, int32_t k, bool* rem) {
@@ -620,11 +627,13 @@ __device__ __forceinline__ void apply_batch(const Inst& in, const uint32_t* e, c
 #pragma unroll
     for (uint32_t r = 0; r < U; ++r) oc[r] = cross[r] ? atomicSub(&in.cnt[u[r]], 1) : 0;
 #pragma unroll
-    for (uint32_t r = 0; r < U; ++r)
-        rem[r] = cross[r] && oc[r] == int32_t(in.lmd) && atomicCAS(&in.idx[u[r]], -1, k) == -1;
-#pragma unroll
-    for (uint32_t r = 0; r < U; ++r)
-        if (rem[r]) clear_alive(in, u[r]);
+    for (uint32_t r = 0; r < U; ++r) {
+        rem[r] = cross[r] && oc[r] == int32_t(in.lmd);
+        if (rem[r]) {
+            *idx_of(in, u[r]) = k;
+            clear_alive(in, u[r]);
+        }
+    }
 }
 
 // Applies every combined decrement in the LDS table and empties it.  All
@@ -740,8 +749,14 @@ __device__ __forceinline__ void cascade_expand(const Inst& in, InstState* st, Co
     __syncthreads();
 }
 
-// Cascade of every instance not done: deferred chunks of this step's list
-// (chunk j goes to block j mod G), then the block's ring until it is empty.
+// Cascade of every instance not done: the chunks of this step's lists, then
+// the block's ring until it is empty.  The chunks of all instances form one
+// sequence (instance after instance) dealt round robin over the blocks, so a
+// step with fewer chunks than blocks gives each block at most one chunk in
+// total instead of one per instance: the tail of a decomposition is a long
+// series of such steps, and each chunk is a chain of dependent memory round
+// trips.  A block without chunks of an instance has nothing in its ring for
+// it either.
 __global__ void __launch_bounds__(kBlock) k_cascade(const Inst* insts, InstState* states, Control* ctl, uint32_t I,
                                                     uint32_t s) {
     __shared__ CascShared sh;
@@ -751,6 +766,7 @@ __global__ void __launch_bounds__(kBlock) k_cascade(const Inst* insts, InstState
         sh.cnt[t] = 0;
     }
     if (threadIdx.x == 0) sh.used = 0;
+    uint64_t seq = 0;  // chunks of the instances before i
     for (uint32_t i = 0; i < I; ++i) {
         InstState* st = &states[i];
         const int32_t phase = st->phase;
@@ -758,6 +774,11 @@ __global__ void __launch_bounds__(kBlock) k_cascade(const Inst* insts, InstState
         const uint32_t par = s & 1u, npar = par ^ 1u;
         const uint64_t pc = st->pcount[par][0];
         if (pc == 0) continue;  // empty frontier: the ring starts empty, nothing to do
+        // chunk j of instance i is item seq + j of the sequence; it goes to
+        // block (seq + j) mod G, so this block's first chunk is j = first
+        const uint64_t first = (uint64_t(b) + G - seq % G) % G;
+        seq += pc;
+        if (first >= pc) continue;
         const Inst& in = insts[i];
         const int32_t k = st->k;
         uint32_t* ring = in.ring + uint64_t(b) * in.cap;
@@ -769,8 +790,8 @@ __global__ void __launch_bounds__(kBlock) k_cascade(const Inst* insts, InstState
         __syncthreads();
         // frontier / deferred chunks
         const uint64_t* pl = defer_list(in, par);
-        for (uint64_t m0 = 0; m0 * G + b < pc; m0 += kBlock) {
-            const uint64_t j = (m0 + threadIdx.x) * G + b;
+        for (uint64_t m0 = 0; m0 * G + first < pc; m0 += kBlock) {
+            const uint64_t j = (m0 + threadIdx.x) * G + first;
             uint64_t start = 0;
             uint32_t len = 0;
             if (j < pc) {
