@@ -107,6 +107,111 @@ def cpu_baseline(spec, kind: str, budget_s: float = 30.0):
             "seconds": round(secs, 4)}
 
 
+def run_batched(steps: int, count: int = 256) -> dict:
+    """Batched form: `count` graphs of the batched workload decomposed in one
+    persistent launch (hg_batch_*, launch shape from its autotuner), against
+    the same graphs decomposed one after another by the single-graph engine
+    (graphs resident, one hg_decompose each).  Results must be identical."""
+    import numpy as np
+
+    from pargridcpu_amd.graph import MultilayerGraph, batch_workload
+    from pargridcpu_amd.hipgrid import Batch, Engine
+
+    t0 = time.perf_counter()
+    gs = [MultilayerGraph.generate(s) for s in batch_workload(count)]
+    D = sum(g.directed_entries() for g in gs)
+    log(f"[bench] batched: generated {count} graphs (D={D}) in {time.perf_counter() - t0:.1f}s")
+    b = Batch(gs)
+    best, tuned_ms = b.tune(reps=3)
+    log(f"[bench] batched: autotuner picked {best} ({tuned_ms:.3f} ms)")
+    walls = []
+    for _ in range(max(steps, 3)):
+        w0 = time.perf_counter()
+        b.decompose()
+        walls.append((time.perf_counter() - w0) * 1e3)
+    dev_ms = b.last_ms()[0]
+    batch_ms = sorted(walls)[len(walls) // 2]
+    engs = [Engine.from_graph(g) for g in gs]
+    for e in engs[:8]:
+        e.decompose()
+    w0 = time.perf_counter()
+    for e in engs:
+        e.decompose()
+    seq_ms = (time.perf_counter() - w0) * 1e3
+    exact = all(np.array_equal(b.core_index(i), e.core_index()) for i, e in enumerate(engs))
+    for e in engs:
+        e.close()
+    b.close()
+    log(f"[bench] batched: launch {batch_ms:.2f} ms wall ({dev_ms:.2f} ms device), sequential {seq_ms:.1f} ms, "
+        f"exact={exact}")
+    return {"workload": f"{count} graphs: n log-U[1e3,2e5], L U{{2..16}}, avg deg U[2,20], gamma U[2.1,3.0]",
+            "graphs": count, "directed_entries": D, "launch_ms": round(batch_ms, 3), "device_ms": round(dev_ms, 3),
+            "graphs_per_s": round(count / (batch_ms * 1e-3), 1), "sequential_ms": round(seq_ms, 3),
+            "sequential_graphs_per_s": round(count / (seq_ms * 1e-3), 1), "speedup": round(seq_ms / batch_ms, 2),
+            "exact": bool(exact), "params": best}
+
+
+def run_serving(n_requests: int = 120, rate: float = 8.0) -> dict:
+    """Serving form: Poisson arrivals at `rate` requests/s, 80% decompositions
+    and 20% densest-subgraph searches; graphs from the batched workload's
+    distribution, 10% of them large (n = 5M, L = 12, 60M sampled edges);
+    200 GB device-memory budget.  The request loop (4 lanes, continuous
+    batching) against one-at-a-time execution on the single-graph engine,
+    same arrival trace; every answer must be identical in both."""
+    import math
+
+    import numpy as np
+
+    from pargridcpu_amd.graph import MultilayerGraph, _unit, batch_spec, large_request_spec
+    from pargridcpu_amd.hipgrid import REQ_DECOMPOSE, REQ_DENSEST, Server
+
+    t0 = time.perf_counter()
+    small = [MultilayerGraph.generate(batch_spec(r, base=0xC5)) for r in range(48)]
+    large = [MultilayerGraph.generate(large_request_spec(r)) for r in range(2)]
+    log(f"[bench] serving: generated the request graphs in {time.perf_counter() - t0:.1f}s")
+    trace, t = [], 0.0
+    for r in range(n_requests):
+        seed = (0xC5 << 32) | r
+        kind = REQ_DENSEST if _unit(seed, 10) < 0.2 else REQ_DECOMPOSE
+        g = large[int(_unit(seed, 12) * len(large))] if _unit(seed, 11) < 0.1 else small[int(_unit(seed, 13) * len(small))]
+        t += -math.log(1.0 - _unit(seed, 14)) / rate
+        trace.append((t, g, kind))
+
+    def serve(**kw):
+        with Server(hbm_budget=200 << 30, **kw) as srv:
+            start = time.perf_counter()
+            ids = []
+            for ta, g, kind in trace:
+                now = time.perf_counter() - start
+                if ta > now:
+                    time.sleep(ta - now)
+                ids.append(srv.submit(g, kind, 1.0))
+            srv.drain()
+            st = srv.stats()
+            infos = [srv.wait(i) for i in ids]
+            outs = [srv.result(i)[0] for i in ids]
+        return st, infos, outs
+
+    res = {}
+    outs0 = None
+    for name, kw in (("continuous", dict(lanes=4, max_batch=64, large_entries=20_000_000)), ("one_at_a_time", dict(plain=True))):
+        st, infos, outs = serve(**kw)
+        sizes = [i.batch_size for i in infos]
+        res[name] = {"p50_ms": round(st["p50_ms"], 2), "p99_ms": round(st["p99_ms"], 2),
+                     "req_per_s": round(n_requests / st["span_s"], 2), "completed": st["completed"],
+                     "failed": st["failed"], "batches": st["batches"], "mean_batch": round(sum(sizes) / len(sizes), 2)}
+        log(f"[bench] serving {name}: {res[name]}")
+        if outs0 is None:
+            outs0 = outs
+        else:
+            res["identical"] = all(np.array_equal(a, c) for a, c in zip(outs0, outs))
+    res["requests"] = n_requests
+    res["offered_rate"] = rate
+    c, p = res["continuous"], res["one_at_a_time"]
+    res["p99_ratio"] = round(p["p99_ms"] / c["p99_ms"], 2) if c["p99_ms"] else None
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -114,6 +219,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default=os.environ.get("HG_WORKLOAD", "c3"))
     ap.add_argument("--cpu-baseline", default="auto", choices=["auto", "reference", "port", "off"])
+    ap.add_argument("--batched", type=int, default=1, help="also run the batched form (0: skip)")
+    ap.add_argument("--serving", type=int, default=1, help="also run the serving form (0: skip)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -182,13 +289,25 @@ def main() -> int:
         except Exception as e:  # reported, never fatal for the GPU number
             cpu = {"value": None, "unit": "GTEPS", "cores": None, "kind": args.cpu_baseline, "sample": f"failed: {e}"}
     eng.close()
+    del eng, g
+    batched = serving = None
+    if args.batched:
+        try:
+            batched = run_batched(args.steps)
+        except Exception as e:  # reported, never fatal for the headline number
+            batched = {"error": str(e)}
+    if args.serving:
+        try:
+            serving = run_serving()
+        except Exception as e:
+            serving = {"error": str(e)}
     line = {
         "metric": METRIC, "value": round(gteps, 4), "unit": "GTEPS", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "int32", "data": "synthetic (seeded Chung-Lu multilayer generator)",
         "config": {"workload": spec.name, "n": n, "L": L, "directed_entries": D, "gamma": spec.gamma, "rho": spec.rho,
                    "offsets": "int64", "parallelism": "single-gpu"},
-        "roofline": roof, "cpu_baseline": cpu,
+        "roofline": roof, "cpu_baseline": cpu, "batched": batched, "serving": serving,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -143,6 +143,135 @@ int hg_graph_destroy(hg_graph* g);
 
 const char* hg_last_error(void);
 
+/* ------------------------------------------------------------------ batches
+ * Many independent graphs decomposed in one persistent launch.  Every
+ * (graph, lmd) pair is a work item in a device queue, ordered by estimated
+ * cost; a fixed grid of workgroups pulls items and each workgroup runs its
+ * item's whole k-chain.  The reference has no batched form (each of its
+ * processes runs one method on one graph, main.cpp:229-257); per item the
+ * algorithm and the results are FCSyncLeft::Execute's (FCSyncLeft.cpp:
+ * 202-252), with the lmd-chains of all graphs run as concurrent tasks the
+ * way ParGridPlus runs one graph's lmd-chains (FCSyncLeft.cpp:371-409).
+ * Results are bit-exact against hg_decompose on each graph alone. */
+typedef struct hg_batch hg_batch;
+
+/* One graph of a batch: its layers as for hg_graph_create.  Limits per
+ * graph: 1 <= L <= 64, 1 <= n, n * L < 2^32, fewer than 2^32 directed
+ * entries. */
+typedef struct hg_batch_graph {
+    const hg_layer_csr* layers;
+    int32_t L;
+    int32_t offset_bits; /* 32 or 64 */
+    int64_t n;
+} hg_batch_graph;
+
+/* Launch shape, chosen by hg_batch_tune or set by the caller. */
+typedef struct hg_batch_params {
+    int32_t block;         /* threads per workgroup: 256, 512 or 1024 */
+    int32_t unroll;        /* entries per thread in flight per cascade round: 2 or 4 */
+    int32_t blocks_per_cu; /* persistent workgroups per compute unit: 1, 2, 4 or 8 */
+    int32_t order;         /* 0: items by estimated cost, largest first; 1: graph order */
+    int32_t use_graph;     /* 1: replay a captured hipGraph; 0: plain launches */
+    int32_t reserved;
+} hg_batch_params;
+
+/* Uploads all graphs (host memory) and allocates the state of every item.
+ * stream: hipStream_t, or NULL for a stream the batch creates. */
+int hg_batch_create(const hg_batch_graph* graphs, int32_t count, void* stream, hg_batch** out);
+
+/* Sets (p != NULL) or resets to defaults (p == NULL) the launch shape. */
+int hg_batch_set_params(hg_batch* b, const hg_batch_params* p);
+int hg_batch_get_params(hg_batch* b, hg_batch_params* p);
+
+/* Autotuner: times every candidate launch shape on this batch (one warm-up
+ * and `reps` timed runs each, median), keeps the fastest and returns it
+ * with its time.  Results are the same for every shape. */
+int hg_batch_tune(hg_batch* b, int32_t reps, hg_batch_params* best, double* best_ms);
+
+/* Decomposes every graph of the batch (all lmd) in one launch. */
+int hg_batch_decompose(hg_batch* b);
+
+/* Results of graph `graph` (0-based) after hg_batch_decompose: core index
+ * [L][n] and cell lengths, as hg_core_index and hg_cells. */
+int hg_batch_core_index(hg_batch* b, int32_t graph, int32_t* core_index);
+int hg_batch_cells(hg_batch* b, int32_t graph, int64_t* cell_len, int32_t cap, int32_t* kmax);
+
+/* Weighted densest subgraph (as hg_densest) of each listed graph, in one
+ * launch; out[i] belongs to graphs[i]. */
+int hg_batch_densest(hg_batch* b, const int32_t* graphs, int32_t count, double beta, hg_wds* out);
+
+/* Device time of the last hg_batch_decompose / hg_batch_densest (ms). */
+int hg_batch_last_ms(hg_batch* b, double* decompose_ms, double* densest_ms);
+
+/* Batch facts: graphs, work items, directed entries, device bytes. */
+int hg_batch_info(hg_batch* b, int32_t* count, int64_t* items, int64_t* directed_entries, int64_t* device_bytes);
+
+int hg_batch_destroy(hg_batch* b);
+
+/* ------------------------------------------------------------------ serving
+ * A request loop over one GPU.  Requests (a graph and "decompose" or
+ * "densest") are copied in at submit and queue in arrival order; `lanes`
+ * worker threads, each with its own HIP stream, take work when idle: a
+ * request with more than `large_entries` directed entries runs alone on the
+ * single-graph engine (hg_graph_*), the others are taken from the head of
+ * the queue, up to `max_batch` at a time, and run as one batch (hg_batch_*).
+ * A request is admitted only while the device bytes of all work in flight
+ * stay within `hbm_budget`.  Requests that arrive while others run join the
+ * next batch a lane forms; the lanes' work overlaps on the device.  Every
+ * answer equals that of hg_graph_* on the request's graph alone.
+ * plain = 1 is the one-at-a-time baseline: one lane, every request alone on
+ * the single-graph engine.  The reference has no serving layer (main.cpp
+ * runs one method on one graph per process). */
+typedef struct hg_server hg_server;
+
+typedef struct hg_server_config {
+    int64_t hbm_budget;     /* device bytes the loop may hold at once */
+    int64_t large_entries;  /* requests above this many directed entries run alone */
+    int32_t lanes;          /* concurrent workers (streams), 1..16 */
+    int32_t max_batch;      /* requests per batch, >= 1 */
+    int32_t plain;          /* 1: one request at a time on the single-graph engine */
+    int32_t reserved;
+    hg_batch_params params; /* launch shape of the batches (block == 0: defaults) */
+} hg_server_config;
+
+#define HG_REQ_DECOMPOSE 0
+#define HG_REQ_DENSEST 1
+
+typedef struct hg_request_info {
+    int32_t status;      /* 0 done, 1 pending, < 0 the HG_ERR_* code it failed with */
+    int32_t kind;        /* HG_REQ_* */
+    int64_t n;
+    int32_t L;
+    int32_t batch_size;  /* requests in the batch it ran in (1: alone) */
+    double queue_ms;     /* submit -> start of its batch */
+    double run_ms;       /* start -> results on the host */
+    double latency_ms;   /* submit -> results on the host */
+    hg_wds wds;          /* HG_REQ_DENSEST: as hg_densest */
+} hg_request_info;
+
+typedef struct hg_server_summary {
+    int64_t submitted, completed, failed, batches;
+    double p50_ms, p99_ms, mean_ms, max_ms; /* latency of completed requests */
+    double span_s;                          /* first submit -> last completion */
+    int64_t peak_bytes;                     /* most device bytes admitted at once */
+} hg_server_summary;
+
+int hg_server_create(const hg_server_config* cfg, hg_server** out);
+/* Queues a request (the graph is copied); *id names it. */
+int hg_server_submit(hg_server* s, const hg_batch_graph* graph, int32_t kind, double beta, int64_t* id);
+/* Blocks until request id is done; fills info (may be NULL). */
+int hg_server_wait(hg_server* s, int64_t id, hg_request_info* info);
+/* Results of a done request: core index [L][n] and kmax [L] (either may be NULL). */
+int hg_server_result(hg_server* s, int64_t id, int32_t* core_index, int32_t* kmax);
+/* Forgets a done request and its results. */
+int hg_server_release(hg_server* s, int64_t id);
+/* Blocks until every submitted request is done. */
+int hg_server_drain(hg_server* s);
+int hg_server_stats(hg_server* s, hg_server_summary* out);
+/* Finishes the queued work, stops the lanes and frees the server. */
+int hg_server_destroy(hg_server* s);
+
+
 #ifdef __cplusplus
 }
 #endif

@@ -431,7 +431,9 @@ int decompose_impl(hg_graph* g) {
     g->timing.launches = int64_t(s) * 3 + 1;
 
     g->h_count.assign(size_t(g->kcap) * L, 0);
-    HIP_TRY(hipMemcpy(g->h_count.data(), g->d_ihist, sizeof(int64_t) * g->h_count.size(), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpyAsync(g->h_count.data(), g->d_ihist, sizeof(int64_t) * g->h_count.size(), hipMemcpyDeviceToHost,
+                           g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
     g->h_kmax.assign(L, 0);
     int64_t total = 0;
     for (uint32_t i = 0; i < L; ++i) {

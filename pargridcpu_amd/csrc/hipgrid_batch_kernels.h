@@ -1,0 +1,446 @@
+// hipgrid_batch_kernels.h -- batched FirmCore: many graphs, one persistent launch.
+//
+// Work model.  A batch holds many independent multilayer graphs resident in
+// HBM.  Every (graph, lmd) pair is one work item; the items sit in a device
+// queue ordered by estimated cost, largest first, and a fixed grid of
+// workgroups pulls them with one atomic per item.  A workgroup runs its
+// item's whole decomposition by itself: level 0, then for k = 0, 1, ... the
+// cascade of level k and the scan of level k+1, until no vertex is alive.
+// Nothing is shared between workgroups except the item counter, so a level
+// needs no grid-wide synchronisation and a launch runs all items of all
+// graphs to completion.  This is the reference's per-cell PeelSync
+// (CoreParallel/FCSyncLeft.cpp:40-131) applied to one (graph, lmd) chain
+// (FCSyncLeft.cpp:155-170 walks the k-chain of one lmd), with the chains of
+// all graphs and all lmd run as concurrent tasks the way ParGridPlus runs
+// its lmd-chains as tasks (FCSyncLeft.cpp:371-409).
+//
+// Per item, in the item's own slices of the state pools:
+//   deg   uint32 [n][L]  residual layer degrees (level 0 pulls them)
+//   cnt   int32 [n]      layers with degree >= the threshold
+//   idx   int32 [n]      core index (internal ids; -1 while alive)
+//   list  uint32 [n]     alive list, compacted in place by every scan
+//   queue uint32 [n]     removal FIFO (each vertex enters once per item)
+//   bits                 liveness bitmap: in LDS when n <= kLdsBits, else
+//                        the item's words in the bitmap pool
+// The same rule as the single-graph engine (FCSyncLeft.cpp:65-108): a scan
+// removes a vertex with fewer than lmd layers of degree >= k+1; in the
+// cascade the decrement taking deg[u][l] from k+1 to k decrements cnt[u],
+// and the one taking cnt[u] from lmd to lmd-1 removes u (every vertex that
+// is not alive holds cnt < lmd, so that decrement is unique).
+//
+// Vertices of each graph are relabelled by total degree, descending, at
+// upload; level 0 gives the heavy head (more than kWaveVertex entries) a
+// wavefront per vertex and the rest a thread per vertex.  Results are
+// written back in the caller's ids at the end of each item.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "hipgrid_kernels.h"
+
+namespace hgb {
+
+using hg::div_L;
+using hg::kWave;
+using hg::lane_id;
+using hg::wave_excl_scan;
+
+constexpr uint32_t kLdsBits = 1u << 18;        // vertices whose liveness bitmap fits in LDS (32 KB)
+constexpr uint32_t kLdsWords = kLdsBits / 32;
+constexpr uint32_t kWaveVertex = 64;          // level 0: vertices with more entries get a wavefront
+
+struct BGraph {
+    uint64_t eb;  // first entry of the graph in adj / adjz
+    uint64_t vb;  // first vertex of the graph in z / perm
+    uint64_t ob;  // first offset of the graph in off (n + 1 entries, local to eb)
+    uint64_t db;  // first word of the graph's deg0 rows [n][L]
+    uint32_t n, L, lmagic, H;  // H: vertices with more than kWaveVertex entries (ids < H)
+};
+
+struct BItem {
+    uint32_t g, lmd;
+    uint64_t degb;  // deg rows [n][L] in the degree pool
+    uint64_t sb;    // n-sized slices: cnt, idx, list, queue pools
+    uint64_t bitb;  // bitmap words in the bitmap pool (used when n > kLdsBits)
+    uint64_t outb;  // core-index row [n] (caller's ids) in out
+    uint64_t histb; // edge histogram [L][kcap] in the histogram pool
+    uint32_t kcap;  // histogram width (> every core index of the graph)
+    uint32_t id;    // item id (slot of its kmax)
+};
+
+// Exclusive prefix sum over a block of B threads; s_tmp needs B/64 + 1 words.
+template <int B>
+__device__ __forceinline__ uint32_t bscan(uint32_t x, uint32_t* total, uint32_t* s_tmp) {
+    constexpr int W = B / kWave;
+    const uint32_t lane = lane_id(), wid = threadIdx.x / kWave;
+    uint32_t wt;
+    const uint32_t wex = wave_excl_scan(x, &wt);
+    if (lane == 0) s_tmp[wid] = wt;
+    __syncthreads();
+    if (wid == 0) {
+        const uint32_t v = lane < uint32_t(W) ? s_tmp[lane] : 0u;
+        uint32_t t;
+        const uint32_t e = wave_excl_scan(v, &t);
+        if (lane < uint32_t(W)) s_tmp[lane] = e;
+        if (lane == 0) s_tmp[W] = t;
+    }
+    __syncthreads();
+    *total = s_tmp[W];
+    const uint32_t r = wex + s_tmp[wid];
+    __syncthreads();
+    return r;
+}
+
+// Degree loads after other threads' atomics: agent-scope loads, not served
+// from a possibly stale vector L1 line.
+__device__ __forceinline__ uint32_t ld_agent(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <int B>
+struct PeelShared {
+    uint32_t bits[kLdsWords];
+    uint32_t pre[B];
+    uint32_t start[B];
+    uint32_t tmp[B / kWave + 1];
+    uint32_t item;
+    uint32_t listlen;
+    uint32_t keep;
+    uint32_t qhead;
+    uint32_t qtail;
+    uint32_t nalive;
+    uint32_t kmax;
+};
+
+struct ItemView {
+    const uint32_t* off;   // local offsets [n+1]
+    const uint32_t* adj;   // entries u*L + l
+    const uint8_t* adjz;   // z(u) per entry
+    const uint8_t* z;      // z per vertex
+    const uint32_t* deg0;  // [n][L]
+    const uint32_t* perm;  // internal id -> caller's id
+    uint32_t* deg;
+    int32_t* cnt;
+    int32_t* idx;
+    uint32_t* list;
+    uint32_t* queue;
+    uint32_t* bits;        // generic pointer: LDS or global
+    bool lds_bits;
+    uint32_t n, L, lmd, magic, H;
+};
+
+__device__ __forceinline__ bool bit_get(const ItemView& v, uint32_t u) {
+    const uint32_t w = v.lds_bits ? v.bits[u >> 5] : __hip_atomic_load(&v.bits[u >> 5], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return (w >> (u & 31)) & 1u;
+}
+__device__ __forceinline__ void bit_clear(const ItemView& v, uint32_t u) { atomicAnd(&v.bits[u >> 5], ~(1u << (u & 31))); }
+
+// Removes u at level k: index, liveness, FIFO.  Called by the one thread
+// whose decrement (or scan) decided the removal.
+template <int B>
+__device__ __forceinline__ void remove_vertex(const ItemView& v, PeelShared<B>* sh, uint32_t u, int32_t k) {
+    v.idx[u] = k;
+    bit_clear(v, u);
+    const uint32_t q = atomicAdd(&sh->qtail, 1u);
+    if (q < v.n) v.queue[q] = u;  // each vertex enters once: q < n always holds
+    atomicSub(&sh->nalive, 1u);
+}
+
+// Level 0.  Rows: deg[v][l] = #{w in N_l(v) : z(w) >= lmd} for vertices with
+// z(v) >= lmd (the others leave at once and no row counts them); cnt = the
+// number of non-empty rows.  Then a vertex is alive iff z(v) >= lmd and
+// cnt >= lmd; the counted ones that are not alive go to the FIFO.
+template <int B>
+__device__ void peel_level0(const ItemView& v, PeelShared<B>* sh) {
+    const uint32_t n = v.n, L = v.L, lmd = v.lmd;
+    const uint32_t lane = lane_id(), wid = threadIdx.x / kWave;
+    constexpr uint32_t W = B / kWave;
+    // heavy head: a wavefront per vertex
+    for (uint32_t u = wid; u < v.H; u += W) {
+        if (v.z[u] < lmd) {
+            if (lane == 0) v.cnt[u] = 0;
+            continue;
+        }
+        uint32_t pos = v.off[u], c1 = 0;
+        for (uint32_t l = 0; l < L; ++l) {
+            const uint32_t d = v.deg0[uint64_t(u) * L + l];
+            uint32_t c = 0;
+            for (uint32_t t = lane; t < d; t += kWave) c += v.adjz[pos + t] >= lmd;
+            for (int o = kWave / 2; o > 0; o >>= 1) c += __shfl_xor(c, o, kWave);
+            pos += d;
+            if (lane == 0) v.deg[uint64_t(u) * L + l] = c;
+            c1 += c >= 1u;
+        }
+        if (lane == 0) v.cnt[u] = int32_t(c1);
+    }
+    // the rest: a thread per vertex
+    for (uint32_t u = v.H + threadIdx.x; u < n; u += B) {
+        uint32_t c1 = 0;
+        if (v.z[u] >= lmd) {
+            uint32_t pos = v.off[u];
+            for (uint32_t l = 0; l < L; ++l) {
+                const uint32_t d = v.deg0[uint64_t(u) * L + l];
+                uint32_t c = 0;
+                for (uint32_t t = 0; t < d; ++t) c += v.adjz[pos + t] >= lmd;
+                pos += d;
+                v.deg[uint64_t(u) * L + l] = c;
+                c1 += c >= 1u;
+            }
+        }
+        v.cnt[u] = int32_t(c1);
+    }
+    __syncthreads();
+    // classification, coalesced
+    for (uint32_t base = 0; base < n; base += B) {
+        const uint32_t u = base + threadIdx.x;
+        const bool valid = u < n;
+        const bool counted = valid && v.z[u] >= lmd;
+        const int32_t c1 = valid ? __hip_atomic_load(&v.cnt[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
+        const bool alive = counted && c1 >= int32_t(lmd);
+        if (valid) v.idx[u] = alive ? -1 : 0;
+        // bitmap words: this wavefront's 64 ids are one aligned run
+        const uint64_t am = __ballot(alive);
+        const uint32_t run0 = base + wid * kWave;
+        if (lane == 0 && run0 < n) {
+            v.bits[run0 >> 5] = uint32_t(am);
+            if (run0 + 32 < n) v.bits[(run0 >> 5) + 1] = uint32_t(am >> 32);
+        }
+        if (am) {
+            uint32_t kb = 0;
+            if (lane == 0) kb = atomicAdd(&sh->listlen, uint32_t(__popcll(am)));
+            kb = __shfl(kb, 0, kWave);
+            if (alive) v.list[kb + __builtin_amdgcn_mbcnt_hi(uint32_t(am >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(am), 0u))] = u;
+        }
+        const uint64_t qm = __ballot(counted && !alive);
+        if (qm) {
+            uint32_t qb = 0;
+            if (lane == 0) qb = atomicAdd(&sh->qtail, uint32_t(__popcll(qm)));
+            qb = __shfl(qb, 0, kWave);
+            if (counted && !alive)
+                v.queue[qb + __builtin_amdgcn_mbcnt_hi(uint32_t(qm >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(qm), 0u))] = u;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) sh->nalive = sh->listlen;
+    __syncthreads();
+}
+
+// Cascade of level k: expands the FIFO from qhead until it is empty, B
+// vertices per round, their entries spread over the block (U per thread in
+// flight).
+template <int B, int U>
+__device__ void peel_cascade(const ItemView& v, PeelShared<B>* sh, int32_t k) {
+    const uint32_t thr = uint32_t(k) + 1u, lmd = v.lmd;
+    for (;;) {
+        __syncthreads();
+        const uint32_t qh = sh->qhead, qt = sh->qtail < v.n ? sh->qtail : v.n;
+        if (qh >= qt) break;
+        const uint32_t nb = qt - qh < uint32_t(B) ? qt - qh : uint32_t(B);
+        uint32_t start = 0, len = 0;
+        if (threadIdx.x < nb) {
+            const uint32_t u = v.queue[qh + threadIdx.x];
+            start = v.off[u];
+            len = v.off[u + 1] - start;
+        }
+        uint32_t total;
+        const uint32_t pre = bscan<B>(len, &total, sh->tmp);
+        sh->pre[threadIdx.x] = pre;
+        sh->start[threadIdx.x] = start;
+        __syncthreads();
+        for (uint32_t j0 = 0; j0 < total; j0 += uint32_t(B) * U) {
+            uint32_t e[U], u[U], old[U];
+            bool live[U];
+#pragma unroll
+            for (int r = 0; r < U; ++r) {
+                const uint32_t j = j0 + uint32_t(r) * B + threadIdx.x;
+                live[r] = j < total;
+                e[r] = 0;
+                if (live[r]) {
+                    uint32_t lo = 0, hi = nb - 1;  // largest p with pre[p] <= j
+                    while (lo < hi) {
+                        const uint32_t mid = (lo + hi + 1) >> 1;
+                        if (sh->pre[mid] <= j) lo = mid;
+                        else hi = mid - 1;
+                    }
+                    const uint32_t p = sh->start[lo] + (j - sh->pre[lo]);
+                    e[r] = v.adj[p];
+                    live[r] = v.adjz[p] >= lmd;
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < U; ++r) {
+                u[r] = div_L(e[r], v.L, v.magic);
+                if (live[r]) live[r] = bit_get(v, u[r]);
+            }
+#pragma unroll
+            for (int r = 0; r < U; ++r) old[r] = live[r] ? atomicSub(&v.deg[e[r]], 1u) : 0u;
+            int32_t oc[U];
+#pragma unroll
+            for (int r = 0; r < U; ++r) oc[r] = (live[r] && old[r] == thr) ? atomicSub(&v.cnt[u[r]], 1) : 0;
+#pragma unroll
+            for (int r = 0; r < U; ++r)
+                if (live[r] && old[r] == thr && oc[r] == int32_t(lmd)) remove_vertex<B>(v, sh, u[r], k);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) sh->qhead = qh + nb;
+    }
+    __syncthreads();
+}
+
+// Scan of level k >= 1 over the alive list, compacted in place.
+template <int B>
+__device__ void peel_scan(const ItemView& v, PeelShared<B>* sh, int32_t k) {
+    const uint32_t thr = uint32_t(k) + 1u, L = v.L, lmd = v.lmd;
+    const uint32_t len = sh->listlen;
+    if (threadIdx.x == 0) sh->keep = 0;
+    __syncthreads();
+    for (uint32_t base = 0; base < len; base += B) {
+        const uint32_t t = base + threadIdx.x;
+        uint32_t u = 0;
+        bool live = false;
+        if (t < len) {
+            u = v.list[t];
+            live = bit_get(v, u);
+        }
+        uint32_t c = 0;
+        if (live)
+            for (uint32_t l = 0; l < L; ++l) c += ld_agent(&v.deg[uint64_t(u) * L + l]) >= thr;
+        const bool rem = live && c < lmd;
+        const bool keep = live && !rem;
+        if (live) v.cnt[u] = int32_t(c);
+        if (rem) remove_vertex<B>(v, sh, u, k);
+        __syncthreads();  // the chunk is read before compaction writes into it
+        const uint64_t km = __ballot(keep);
+        if (km) {
+            uint32_t kb = 0;
+            if (lane_id() == 0) kb = atomicAdd(&sh->keep, uint32_t(__popcll(km)));
+            kb = __shfl(kb, 0, kWave);
+            if (keep) v.list[kb + __builtin_amdgcn_mbcnt_hi(uint32_t(km >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(km), 0u))] = u;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) sh->listlen = sh->keep;
+    __syncthreads();
+}
+
+// Resets the item counter (a kernel, so that a captured graph holds no
+// memset node).
+__global__ void k_batch_reset(uint32_t* head) {
+    if (threadIdx.x == 0) head[0] = 0;
+}
+
+template <int B, int U>
+__global__ void __launch_bounds__(B) k_batch_peel(const BGraph* graphs, const BItem* items, uint32_t nitems,
+                                                  uint32_t* head, const uint32_t* off, const uint32_t* adj,
+                                                  const uint8_t* adjz, const uint8_t* z, const uint32_t* deg0,
+                                                  const uint32_t* perm, uint32_t* degpool, int32_t* cntpool,
+                                                  int32_t* idxpool, uint32_t* listpool, uint32_t* qpool,
+                                                  uint32_t* bitpool, int32_t* out, int32_t* kmax_out) {
+    __shared__ PeelShared<B> sh;
+    for (;;) {
+        if (threadIdx.x == 0) sh.item = atomicAdd(head, 1u);
+        __syncthreads();
+        const uint32_t it = sh.item;
+        if (it >= nitems) break;
+        const BItem item = items[it];
+        const BGraph gr = graphs[item.g];
+        ItemView v;
+        v.off = off + gr.ob;
+        v.adj = adj + gr.eb;
+        v.adjz = adjz + gr.eb;
+        v.z = z + gr.vb;
+        v.deg0 = deg0 + gr.db;
+        v.perm = perm + gr.vb;
+        v.deg = degpool + item.degb;
+        v.cnt = cntpool + item.sb;
+        v.idx = idxpool + item.sb;
+        v.list = listpool + item.sb;
+        v.queue = qpool + item.sb;
+        v.lds_bits = gr.n <= kLdsBits;
+        v.bits = v.lds_bits ? sh.bits : bitpool + item.bitb;
+        v.n = gr.n;
+        v.L = gr.L;
+        v.lmd = item.lmd;
+        v.magic = gr.lmagic;
+        v.H = gr.H;
+        if (threadIdx.x == 0) {
+            sh.listlen = 0;
+            sh.qhead = 0;
+            sh.qtail = 0;
+            sh.kmax = 0;
+        }
+        __syncthreads();
+        peel_level0<B>(v, &sh);
+        // every vertex is removed by level max-degree + 1 at the latest
+        // (item.kcap = max layer degree + 2), which also bounds the loop
+        int32_t k = 0;
+        for (;;) {
+            peel_cascade<B, U>(v, &sh, k);
+            if (sh.nalive == 0 || uint32_t(k) + 1 >= item.kcap) break;
+            ++k;
+            peel_scan<B>(v, &sh, k);
+        }
+        // every vertex got its index at level <= k; k is the largest index
+        for (uint32_t u = threadIdx.x; u < v.n; u += B) out[item.outb + v.perm[u]] = v.idx[u];
+        if (threadIdx.x == 0) kmax_out[item.id] = k;
+        __syncthreads();
+    }
+}
+
+// Edge histogram of the listed items (a block per item, round robin):
+// hist[l][m] += 1 for every entry v -> (u, l) with m = min(idx v, idx u) >= 1.
+template <int B>
+__global__ void __launch_bounds__(B) k_batch_edge_hist(const BGraph* graphs, const BItem* items, uint32_t nsel,
+                                                       const uint32_t* off, const uint32_t* adj,
+                                                       const int32_t* idxpool, uint32_t* histpool) {
+    __shared__ uint32_t s_pre[B];
+    __shared__ uint32_t s_start[B];
+    __shared__ int32_t s_iv[B];
+    __shared__ uint32_t s_tmp[B / kWave + 1];
+    for (uint32_t si = blockIdx.x; si < nsel; si += gridDim.x) {
+        const BItem item = items[si];
+        const BGraph gr = graphs[item.g];
+        const uint32_t* goff = off + gr.ob;
+        const uint32_t* gadj = adj + gr.eb;
+        const int32_t* idx = idxpool + item.sb;
+        uint32_t* hist = histpool + item.histb;
+        for (uint32_t v0 = 0; v0 < gr.n; v0 += B) {
+            const uint32_t vv = v0 + threadIdx.x;
+            uint32_t start = 0, len = 0;
+            int32_t iv = 0;
+            if (vv < gr.n) {
+                iv = idx[vv];
+                if (iv >= 1) {
+                    start = goff[vv];
+                    len = goff[vv + 1] - start;
+                }
+            }
+            uint32_t total;
+            const uint32_t pre = bscan<B>(len, &total, s_tmp);
+            s_pre[threadIdx.x] = pre;
+            s_start[threadIdx.x] = start;
+            s_iv[threadIdx.x] = iv;
+            __syncthreads();
+            for (uint32_t j = threadIdx.x; j < total; j += B) {
+                uint32_t lo = 0, hi = B - 1;
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi + 1) >> 1;
+                    if (s_pre[mid] <= j) lo = mid;
+                    else hi = mid - 1;
+                }
+                const uint32_t e = gadj[s_start[lo] + (j - s_pre[lo])];
+                const uint32_t u = div_L(e, gr.L, gr.lmagic);
+                const uint32_t l = e - u * gr.L;
+                const int32_t iu = idx[u];
+                if (iu >= 1) {
+                    const uint32_t m = uint32_t(iu < s_iv[lo] ? iu : s_iv[lo]);
+                    atomicAdd(&hist[uint64_t(l) * item.kcap + m], 1u);
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+}  // namespace hgb

@@ -59,6 +59,9 @@
 #include <stdint.h>
 
 namespace hg {
+// Internal linkage: each translation unit of the library that includes this
+// header gets its own copy of the kernels it launches.
+namespace {
 
 constexpr int kWave = 64;
 constexpr int kBlock = 512;  // threads per block of the level kernels (8 wavefronts)
@@ -962,4 +965,5 @@ __global__ void k_cell_score(const uint64_t* hist, uint32_t kcap, uint32_t L, ui
     sel[k] = bsel;
 }
 
+}  // namespace
 }  // namespace hg

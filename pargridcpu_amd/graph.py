@@ -202,3 +202,45 @@ C1 = GraphSpec("c1-n20k-L7", 0xC1, 20_000, 7, 2.5, 0.5, 80_000, 80_000, 0.0, ord
 C2 = GraphSpec("c2-n1M-L10", 0xC2, 1_000_000, 10, 2.2, 0.5, 300_000, 3_000_000, 0.0)
 C3 = GraphSpec("c3-n100M-L8", 0xC3, 100_000_000, 8, 2.1, 0.3, 150_000_000, 150_000_000, 200_000.0)
 WORKLOADS = {"c1": C1, "c2": C2, "c3": C3}
+
+_MASK64 = (1 << 64) - 1
+
+
+def _splitmix64(seed: int, t: int) -> int:
+    """Output t of the SplitMix64 stream seeded with seed."""
+    z = (seed + (t + 1) * 0x9E3779B97F4A7C15) & _MASK64
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _MASK64
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _MASK64
+    return z ^ (z >> 31)
+
+
+def _unit(seed: int, t: int) -> float:
+    return (_splitmix64(seed, t) >> 11) * (1.0 / (1 << 53))
+
+
+def batch_spec(g: int, base: int = 0xC4, n_lo: int = 1_000, n_hi: int = 200_000) -> GraphSpec:
+    """Graph g of the batched workload: n log-uniform in [n_lo, n_hi], L
+    uniform in {2..16}, average per-layer degree uniform in [2, 20], gamma
+    uniform in [2.1, 3.0], 50% per-layer weight reshuffle; seed base<<32 | g
+    draws the parameters (outputs 0..3 of its SplitMix64 stream) and seeds
+    the generator."""
+    import math
+
+    seed = (base << 32) | g
+    n = int(round(math.exp(math.log(n_lo) + _unit(seed, 0) * (math.log(n_hi) - math.log(n_lo)))))
+    L = 2 + min(14, int(_unit(seed, 1) * 15))
+    avg_deg = 2.0 + 18.0 * _unit(seed, 2)
+    gamma = 2.1 + 0.9 * _unit(seed, 3)
+    m = max(1, int(round(n * avg_deg / 2)))
+    return GraphSpec(f"batch{base:x}-{g}", seed, n, L, gamma, 0.5, m, m, 0.0)
+
+
+def batch_workload(count: int = 256, base: int = 0xC4) -> list[GraphSpec]:
+    """The batched workload: `count` independent graphs (batch_spec)."""
+    return [batch_spec(g, base) for g in range(count)]
+
+
+def large_request_spec(r: int, base: int = 0xC5) -> GraphSpec:
+    """The serving workload's large request: n = 5M, L = 12, 60M edges (5M
+    sampled per layer), gamma 2.2, 50% reshuffle."""
+    return GraphSpec(f"large{base:x}-{r}", (base << 32) | r, 5_000_000, 12, 2.2, 0.5, 5_000_000, 5_000_000, 0.0)

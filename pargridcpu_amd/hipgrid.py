@@ -49,6 +49,41 @@ class Timing(C.Structure):
                 ("launches", C.c_int64)]
 
 
+class BatchGraph(C.Structure):
+    _fields_ = [("layers", C.c_void_p), ("L", C.c_int32), ("offset_bits", C.c_int32), ("n", C.c_int64)]
+
+
+class BatchParams(C.Structure):
+    _fields_ = [("block", C.c_int32), ("unroll", C.c_int32), ("blocks_per_cu", C.c_int32), ("order", C.c_int32),
+                ("use_graph", C.c_int32), ("reserved", C.c_int32)]
+
+    def as_dict(self) -> dict:
+        return {f: int(getattr(self, f)) for f, _ in self._fields_ if f != "reserved"}
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget", C.c_int64), ("large_entries", C.c_int64), ("lanes", C.c_int32), ("max_batch", C.c_int32),
+                ("plain", C.c_int32), ("reserved", C.c_int32), ("params", BatchParams)]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [("status", C.c_int32), ("kind", C.c_int32), ("n", C.c_int64), ("L", C.c_int32),
+                ("batch_size", C.c_int32), ("queue_ms", C.c_double), ("run_ms", C.c_double),
+                ("latency_ms", C.c_double), ("wds", Wds)]
+
+
+class ServerSummary(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("failed", C.c_int64), ("batches", C.c_int64),
+                ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("mean_ms", C.c_double), ("max_ms", C.c_double),
+                ("span_s", C.c_double), ("peak_bytes", C.c_int64)]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+REQ_DECOMPOSE = 0
+REQ_DENSEST = 1
+
 _lib = None
 
 # Every symbol include/hipgrid.h declares, with its ctypes signature.
@@ -68,6 +103,26 @@ SIGNATURES = {
                       C.c_int),
     "hg_graph_destroy": ([_P], C.c_int),
     "hg_last_error": ([], C.c_char_p),
+    "hg_batch_create": ([C.POINTER(BatchGraph), C.c_int32, _P, C.POINTER(_P)], C.c_int),
+    "hg_batch_set_params": ([_P, C.POINTER(BatchParams)], C.c_int),
+    "hg_batch_get_params": ([_P, C.POINTER(BatchParams)], C.c_int),
+    "hg_batch_tune": ([_P, C.c_int32, C.POINTER(BatchParams), C.POINTER(C.c_double)], C.c_int),
+    "hg_batch_decompose": ([_P], C.c_int),
+    "hg_batch_core_index": ([_P, C.c_int32, C.POINTER(C.c_int32)], C.c_int),
+    "hg_batch_cells": ([_P, C.c_int32, C.POINTER(C.c_int64), C.c_int32, C.POINTER(C.c_int32)], C.c_int),
+    "hg_batch_densest": ([_P, C.POINTER(C.c_int32), C.c_int32, C.c_double, C.POINTER(Wds)], C.c_int),
+    "hg_batch_last_ms": ([_P, C.POINTER(C.c_double), C.POINTER(C.c_double)], C.c_int),
+    "hg_batch_info": ([_P, C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)],
+                      C.c_int),
+    "hg_batch_destroy": ([_P], C.c_int),
+    "hg_server_create": ([C.POINTER(ServerConfig), C.POINTER(_P)], C.c_int),
+    "hg_server_submit": ([_P, C.POINTER(BatchGraph), C.c_int32, C.c_double, C.POINTER(C.c_int64)], C.c_int),
+    "hg_server_wait": ([_P, C.c_int64, C.POINTER(RequestInfo)], C.c_int),
+    "hg_server_result": ([_P, C.c_int64, C.POINTER(C.c_int32), C.POINTER(C.c_int32)], C.c_int),
+    "hg_server_release": ([_P, C.c_int64], C.c_int),
+    "hg_server_drain": ([_P], C.c_int),
+    "hg_server_stats": ([_P, C.POINTER(ServerSummary)], C.c_int),
+    "hg_server_destroy": ([_P], C.c_int),
 }
 
 
@@ -246,4 +301,184 @@ class HipGrid:
         return np.float32(w.max_density), int(w.res_k), int(w.res_lmd), int(w.res_len), verts
 
 
-__all__ = ["Engine", "FCCoreTree", "HipGrid", "HgError", "NativeLibraryMissing", "SIGNATURES"]
+def _graph_desc(g, offset_bits: int = 64):
+    """(BatchGraph, arrays to keep alive) for an object with .layers() and .n."""
+    want = np.int64 if offset_bits == 64 else np.int32
+    layers, n = (g.layers(), g.n) if hasattr(g, "layers") else g
+    keep = []
+    arr = (LayerCSR * max(len(layers), 1))()
+    for l, lay in enumerate(layers):
+        rp = np.ascontiguousarray(lay.row_ptr, dtype=want)
+        col = np.ascontiguousarray(lay.col, dtype=np.uint32)
+        keep += [rp, col]
+        arr[l].row_ptr = rp.ctypes.data
+        arr[l].col = col.ctypes.data if col.size else rp.ctypes.data
+    keep.append(arr)
+    d = BatchGraph()
+    d.layers = C.cast(arr, C.c_void_p)
+    d.L = len(layers)
+    d.offset_bits = offset_bits
+    d.n = int(n)
+    return d, keep
+
+
+class Batch:
+    """Many graphs decomposed in one persistent launch (hg_batch handle).
+
+    graphs: objects with .layers() (CSR layers in -o order) and .n, or
+    (layers, n) pairs."""
+
+    def __init__(self, graphs, stream: int | None = None, offset_bits: int = 64):
+        keep = []
+        descs = (BatchGraph * max(len(graphs), 1))()
+        self.shapes = []
+        for i, g in enumerate(graphs):
+            d, k = _graph_desc(g, offset_bits)
+            keep.append(k)
+            descs[i] = d
+            self.shapes.append((int(d.n), int(d.L)))
+        h = _P()
+        _check(lib().hg_batch_create(descs, len(graphs), _P(stream) if stream else None, C.byref(h)))
+        self._h = h
+        self.count = len(graphs)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            _check(lib().hg_batch_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_params(self, **kw) -> None:
+        p = self.params()
+        for k, v in kw.items():
+            setattr(p, k, int(v))
+        _check(lib().hg_batch_set_params(self._h, C.byref(p)))
+
+    def params(self) -> BatchParams:
+        p = BatchParams()
+        _check(lib().hg_batch_get_params(self._h, C.byref(p)))
+        return p
+
+    def tune(self, reps: int = 3):
+        p, ms = BatchParams(), C.c_double()
+        _check(lib().hg_batch_tune(self._h, int(reps), C.byref(p), C.byref(ms)))
+        return p.as_dict(), ms.value
+
+    def decompose(self) -> float:
+        """Runs the launch; returns its device time in ms."""
+        _check(lib().hg_batch_decompose(self._h))
+        return self.last_ms()[0]
+
+    def last_ms(self):
+        a, b = C.c_double(), C.c_double()
+        _check(lib().hg_batch_last_ms(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def core_index(self, g: int) -> np.ndarray:
+        n, L = self.shapes[g]
+        out = np.empty((L, n), dtype=np.int32)
+        _check(lib().hg_batch_core_index(self._h, int(g), _ptr(out, C.c_int32)))
+        return out
+
+    def cells(self, g: int, cap: int | None = None):
+        n, L = self.shapes[g]
+        kmax = np.zeros(L, dtype=np.int32)
+        _check(lib().hg_batch_cells(self._h, int(g), None, 0, _ptr(kmax, C.c_int32)))
+        if cap is None:
+            cap = int(kmax.max()) + 2
+        cl = np.zeros((L, cap), dtype=np.int64)
+        _check(lib().hg_batch_cells(self._h, int(g), _ptr(cl, C.c_int64), cap, None))
+        return cl, kmax
+
+    def densest(self, graphs, beta: float = 1.0) -> list:
+        ids = np.ascontiguousarray(list(graphs), dtype=np.int32)
+        out = (Wds * max(len(ids), 1))()
+        _check(lib().hg_batch_densest(self._h, _ptr(ids, C.c_int32), len(ids), float(beta), out))
+        return [out[i] for i in range(len(ids))]
+
+    def info(self) -> dict:
+        c, it, D, b = C.c_int32(), C.c_int64(), C.c_int64(), C.c_int64()
+        _check(lib().hg_batch_info(self._h, C.byref(c), C.byref(it), C.byref(D), C.byref(b)))
+        return {"graphs": c.value, "items": it.value, "directed_entries": D.value, "device_bytes": b.value}
+
+
+class Server:
+    """Request loop over one GPU (hg_server handle): submit graphs for
+    decomposition or densest-subgraph search, wait for their answers."""
+
+    def __init__(self, hbm_budget: int = 200 << 30, lanes: int = 4, max_batch: int = 64,
+                 large_entries: int = 50_000_000, plain: bool = False, params: dict | None = None):
+        cfg = ServerConfig()
+        cfg.hbm_budget = int(hbm_budget)
+        cfg.large_entries = int(large_entries)
+        cfg.lanes = int(lanes)
+        cfg.max_batch = int(max_batch)
+        cfg.plain = 1 if plain else 0
+        for k, v in (params or {}).items():
+            setattr(cfg.params, k, int(v))
+        h = _P()
+        _check(lib().hg_server_create(C.byref(cfg), C.byref(h)))
+        self._h = h
+        self._shapes = {}
+
+    def submit(self, g, kind: int = REQ_DECOMPOSE, beta: float = 1.0) -> int:
+        d, keep = _graph_desc(g)
+        rid = C.c_int64()
+        _check(lib().hg_server_submit(self._h, C.byref(d), int(kind), float(beta), C.byref(rid)))
+        self._shapes[rid.value] = (int(d.n), int(d.L))
+        return rid.value
+
+    def wait(self, rid: int) -> RequestInfo:
+        info = RequestInfo()
+        _check(lib().hg_server_wait(self._h, int(rid), C.byref(info)))
+        return info
+
+    def result(self, rid: int):
+        n, L = self._shapes[rid]
+        ci = np.empty((L, n), dtype=np.int32)
+        kmax = np.empty(L, dtype=np.int32)
+        _check(lib().hg_server_result(self._h, int(rid), _ptr(ci, C.c_int32), _ptr(kmax, C.c_int32)))
+        return ci, kmax
+
+    def release(self, rid: int) -> None:
+        _check(lib().hg_server_release(self._h, int(rid)))
+        self._shapes.pop(rid, None)
+
+    def drain(self) -> None:
+        _check(lib().hg_server_drain(self._h))
+
+    def stats(self) -> dict:
+        s = ServerSummary()
+        _check(lib().hg_server_stats(self._h, C.byref(s)))
+        return s.as_dict()
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            _check(lib().hg_server_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+__all__ = ["Batch", "Engine", "Server", "FCCoreTree", "HipGrid", "HgError", "NativeLibraryMissing", "SIGNATURES"]

@@ -51,6 +51,19 @@ def test_create_argument_errors_without_device():
     assert lib.hg_graph_destroy(None) == -1
 
 
+def test_batch_argument_errors_without_device():
+    lib = hg.lib()
+    out = C.c_void_p()
+    assert lib.hg_batch_create(None, 1, None, C.byref(out)) == -1
+    descs = (hg.BatchGraph * 1)()
+    assert lib.hg_batch_create(descs, 0, None, C.byref(out)) == -1
+    assert b"count" in lib.hg_last_error()
+    assert lib.hg_batch_decompose(None) == -1
+    assert lib.hg_batch_destroy(None) == -1
+    p = hg.BatchParams()
+    assert lib.hg_batch_set_params(None, C.byref(p)) == -1
+
+
 def test_driver_flags():
     exe = os.path.join(ROOT, "pargridcpu_amd", "bin", "hipgrid")
     assert os.path.exists(exe)

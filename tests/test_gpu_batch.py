@@ -1,0 +1,133 @@
+"""GPU tests of batched decomposition (hg_batch_*): every graph of a batch
+gets exactly the results of the oracle and of the single-graph engine on
+that graph alone, for every launch shape, on every replay of the captured
+launch."""
+import ctypes as C
+import dataclasses
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from pargridcpu_amd.graph import MultilayerGraph, batch_spec
+from pargridcpu_amd.hipgrid import Batch, Engine, HgError, lib
+from tests import graphs as G
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg: str) -> None:
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def mixed_graphs(count=24, n_hi=6000, base=0xB7):
+    """Seeded graphs of the batched workload's shape distribution at small n,
+    plus hand-built edge cases."""
+    gs = [MultilayerGraph.generate(batch_spec(g, base=base, n_lo=200, n_hi=n_hi)) for g in range(count)]
+    n = 40
+    es = G.clique(range(6)) + G.star(6, range(7, 15)) + G.cycle(range(15, 25)) + G.path(range(25, 33))
+    gs.append(G.graph(n, [es, es, es]))                        # identical layers
+    gs.append(G.graph(10, [[], []]))                           # no edges at all
+    gs.append(G.graph(1, [[]]))                                # one vertex
+    gs.append(G.graph(5001, [G.star(0, range(1, 5001))] * 2))  # one hub
+    gs.append(G.graph(10, [G.clique(range(5)), G.clique(range(5, 10))]))  # disjoint layers
+    return gs
+
+
+def engine_results(mg, beta=None):
+    with Engine.from_graph(mg) as eng:
+        eng.decompose()
+        ci = eng.core_index()
+        cl, kmax = eng.cells()
+        w = eng.densest(beta) if beta is not None else None
+    return ci, cl, kmax, w
+
+
+def test_batch_matches_oracle_and_engine():
+    gs = mixed_graphs()
+    say(f"case batch mixed: {len(gs)} graphs")
+    with Batch(gs) as b:
+        info = b.info()
+        assert info["graphs"] == len(gs)
+        b.decompose()
+        for i, mg in enumerate(gs):
+            ci = b.core_index(i)
+            ref = O.coreindex(mg.n, mg.layers())
+            assert np.array_equal(ci, ref), f"graph {i}"
+            cl, kmax = b.cells(i)
+            assert np.array_equal(kmax, ref.max(axis=1) if mg.n else kmax)
+            eci, ecl, ekmax, _ = engine_results(mg)
+            assert np.array_equal(ci, eci)
+            assert np.array_equal(kmax, ekmax)
+            w = min(cl.shape[1], ecl.shape[1])
+            assert np.array_equal(cl[:, :w], ecl[:, :w])
+
+
+@pytest.mark.parametrize("block,unroll,bpc,order,use_graph", [(256, 2, 1, 0, 1), (512, 4, 2, 1, 1), (1024, 4, 1, 0, 0),
+                                                               (256, 4, 8, 1, 0)])
+def test_launch_shapes_agree(block, unroll, bpc, order, use_graph):
+    gs = mixed_graphs(count=12, base=0xB8)
+    refs = [O.coreindex(mg.n, mg.layers()) for mg in gs]
+    say(f"case batch shape block={block} unroll={unroll} bpc={bpc} order={order} graph={use_graph}")
+    with Batch(gs) as b:
+        b.set_params(block=block, unroll=unroll, blocks_per_cu=bpc, order=order, use_graph=use_graph)
+        for rep in range(3):  # replays of the captured launch, with host work between them
+            b.decompose()
+            for i in range(len(gs)):
+                assert np.array_equal(b.core_index(i), refs[i]), f"replay {rep} graph {i}"
+
+
+def test_large_graph_global_bitmap():
+    """n above the LDS bitmap capacity (262144 vertices): the item keeps its
+    liveness bits in global memory."""
+    spec = dataclasses.replace(batch_spec(3, base=0xB9), n=300_000, m_lo=600_000, m_hi=600_000, L=3)
+    gs = [MultilayerGraph.generate(spec), MultilayerGraph.generate(batch_spec(1, base=0xB9, n_hi=3000))]
+    say("case batch large graph n=300000")
+    with Batch(gs) as b:
+        b.decompose()
+        for i, mg in enumerate(gs):
+            assert np.array_equal(b.core_index(i), O.coreindex(mg.n, mg.layers()))
+
+
+@pytest.mark.parametrize("beta", [1.0, 0.5, 2.0])
+def test_batch_densest(beta):
+    gs = mixed_graphs(count=10, base=0xBA)
+    say(f"case batch densest beta={beta}")
+    with Batch(gs) as b:
+        b.decompose()
+        sel = list(range(len(gs)))
+        ws = b.densest(sel, beta)
+        for i in sel:
+            _, _, _, w = engine_results(gs[i], beta)
+            got = ws[i]
+            assert np.float32(got.max_density) == np.float32(w.max_density), f"graph {i}"
+            assert (got.res_k, got.res_lmd, got.res_len) == (w.res_k, w.res_lmd, w.res_len), f"graph {i}"
+            ref = O.wds(gs[i].n, gs[i].layers(), beta)
+            assert np.float32(got.max_density) == ref["max_density"]
+
+
+def test_tune_keeps_results():
+    gs = mixed_graphs(count=8, base=0xBB)
+    refs = [O.coreindex(mg.n, mg.layers()) for mg in gs]
+    say("case batch tune")
+    with Batch(gs) as b:
+        best, ms = b.tune(reps=1)
+        assert best["block"] in (256, 512, 1024) and ms > 0
+        b.decompose()
+        for i in range(len(gs)):
+            assert np.array_equal(b.core_index(i), refs[i])
+
+
+def test_batch_errors():
+    gs = mixed_graphs(count=2, base=0xBC)
+    with Batch(gs) as b:
+        with pytest.raises(HgError) as e:
+            b.core_index(0)  # before decompose
+        assert e.value.code == -5
+        with pytest.raises(HgError):
+            b.set_params(block=300)
+        b.decompose()
+        out = np.empty(16, dtype=np.int32)
+        assert lib().hg_batch_core_index(b._h, len(gs), out.ctypes.data_as(C.POINTER(C.c_int32))) == -1
