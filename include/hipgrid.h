@@ -115,6 +115,19 @@ typedef struct hg_wds {
 } hg_wds;
 int hg_densest(hg_graph* g, double beta, hg_wds* out);
 
+/* -m WdsCoreIndex (CoreIndex::WdsCoreIndex, CoreIndexParallel/CoreIndex.cpp:
+ * 307-478; main.cpp:444-451): the same cells scored with that routine's
+ * float arithmetic (float edge counters, which stop at 2^24; float beta;
+ * float densities).  per_lmd[lmd-1] (may be NULL) is the best cell of each
+ * lmd, k ascending, updated on >=.  *answer (may be NULL) is what the
+ * reference prints: the first record with the largest density among
+ * records 0..L-1, where record 0 is all zeros (the reference leaves it
+ * unwritten) and record lmd is per_lmd[lmd-1] -- so lmd = L never takes
+ * part, and for L = 1 the answer is record 0.  res_len is |S(res_k,
+ * res_lmd)|; num_select_layer the layer count of the best density.
+ * Requires hg_decompose first. */
+int hg_densest_coreindex(hg_graph* g, double beta, hg_wds* answer, hg_wds* per_lmd);
+
 /* Per-cell densities of the last hg_densest: density[(lmd-1)*cap + k] for
  * k < cap (0 where the cell is empty or k = 0).  Either output may be NULL. */
 int hg_cell_density(hg_graph* g, double* density, int32_t* num_select_layer, int32_t cap);

@@ -26,6 +26,8 @@
  *   orc_bruteforce: definition-level fixed point, checked by the closed forms.
  *   orc_wds: against the reference's printed WdsList result (golden), plus a
  *     brute-force per-layer induced edge count on small graphs.
+ *   orc_wds_coreindex: against get_wds's per-lmd results from the reference
+ *     built from source (golden), plus an independent numpy recomputation.
  */
 #include <math.h>
 #include <stdint.h>
@@ -517,6 +519,72 @@ done:
     }
     free(degs); free(iv); free(bucket_of); free(in_n); free(nlist);
     return rc;
+}
+
+/* ------------------------------------------------------------ WdsCoreIndex
+ * Per lmd, the result get_wds leaves in its MaxDensity record
+ * (CoreIndexParallel/CoreIndex.cpp:347-413): for k = 1, 2, ... while
+ * S(k, lmd) = {v : index >= k} is not empty, calculateWeightDenestSubgraph-
+ * Common (:307-345) scores the cell in float: per layer a float counter
+ * incremented once per directed entry inside S, halved; sorted descending;
+ * density_l = (float)((degree_l / |S|) * pow(l+1, beta)) with the float
+ * beta promoted to double; the record takes (density, k, lmd) on >=.
+ * The record starts at {0, 0, 0.0f}.  Indices come from orc_coreindex. */
+static int cmp_desc_float(const void* a, const void* b) {
+    const float x = *(const float*)a, y = *(const float*)b;
+    return x < y ? 1 : (x > y ? -1 : 0);
+}
+
+int orc_wds_coreindex(int64_t n, int32_t L, const int64_t* const* row_ptr, const uint32_t* const* col, float beta,
+                      float* density, uint32_t* res_k, uint32_t* res_lmd) {
+    if (n <= 0 || L <= 0 || L > 64) return -1;
+    orc_graph g = {n, L, row_ptr, col};
+    int32_t* ci = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n * L));
+    if (!ci) return -3;
+    int rc = orc_coreindex(n, L, row_ptr, col, ci);
+    if (rc) {
+        free(ci);
+        return rc;
+    }
+    for (int32_t lmd = 1; lmd <= L; ++lmd) {
+        const int32_t* idx = ci + (size_t)(lmd - 1) * (size_t)n;
+        float best = 0.0f;
+        uint32_t bk = 0, bl = 0;
+        int32_t kmax = 0;
+        for (int64_t v = 0; v < n; ++v)
+            if (idx[v] > kmax) kmax = idx[v];
+        for (int32_t k = 1; k <= kmax; ++k) {
+            uint32_t n_node = 0;
+            for (int64_t v = 0; v < n; ++v) n_node += idx[v] >= k;
+            float deg[64];
+            for (int l = 0; l < L; ++l) {
+                float n_edge = 0;
+                for (int64_t v = 0; v < n; ++v) {
+                    if (idx[v] < k) continue;
+                    const uint32_t* nb = g.col[l] + g.row_ptr[l][v];
+                    const int64_t d = deg_of(&g, l, v);
+                    for (int64_t t = 0; t < d; ++t)
+                        if (idx[nb[t]] >= k) n_edge++;
+                }
+                deg[l] = n_edge / 2;
+            }
+            qsort(deg, (size_t)L, sizeof(float), cmp_desc_float);
+            for (int l = 0; l < L; ++l) {
+                const float mad = deg[l] / n_node;
+                const float dens = (float)(mad * pow((double)(l + 1), (double)beta));
+                if (dens >= best) {
+                    best = dens;
+                    bk = (uint32_t)k;
+                    bl = (uint32_t)lmd;
+                }
+            }
+        }
+        density[lmd - 1] = best;
+        res_k[lmd - 1] = bk;
+        res_lmd[lmd - 1] = bl;
+    }
+    free(ci);
+    return 0;
 }
 
 /* ------------------------------------------------------------- brute force

@@ -121,3 +121,36 @@ def wds(n, layers, beta=1.0, cap=None):
         raise RuntimeError(f"orc_wds failed {rc}")
     return {"max_density": np.float32(md.value), "res_k": rk.value, "res_lmd": rl.value, "res_len": rn.value,
             "cell_density": dens, "cell_sel": sel}
+
+
+def wds_coreindex(n, layers, beta=1.0):
+    """WdsCoreIndex restatement: per-lmd (density float32, res_k, res_lmd) as
+    get_wds leaves them, and the answer CoreIndex::WdsCoreIndex prints."""
+    L = len(layers)
+    RP, CL, keep = _layer_ptrs(layers)
+    dens = np.zeros(L, dtype=np.float32)
+    rk = np.zeros(L, dtype=np.uint32)
+    rl = np.zeros(L, dtype=np.uint32)
+    rc = lib().orc_wds_coreindex(C.c_int64(n), C.c_int32(L), RP, CL, C.c_float(beta),
+                                 dens.ctypes.data_as(C.POINTER(C.c_float)), rk.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                 rl.ctypes.data_as(C.POINTER(C.c_uint32)))
+    if rc != 0:
+        raise RuntimeError(f"orc_wds_coreindex failed {rc}")
+    return {"density": dens, "res_k": rk, "res_lmd": rl, "answer": wds_coreindex_answer(dens, rk, rl)}
+
+
+def wds_coreindex_answer(dens, rk, rl):
+    """CoreIndex::WdsCoreIndex's printed answer (CoreIndexParallel/CoreIndex.cpp:
+    448-478): records [0..L] with record lmd = get_wds's result for lmd and
+    record 0 never written; std::sort of records [0, L) by density,
+    descending, prints record 0 after the sort.  So lmd = L takes no part,
+    and with L <= 16 (insertion sort, stable) the first record with the
+    largest density wins.  Record 0 is read uninitialised by the reference;
+    it is taken as zeros here (for L = 1 the reference prints garbage)."""
+    recs = [(np.float32(0.0), 0, 0)] + [(np.float32(dens[i]), int(rk[i]), int(rl[i])) for i in range(len(dens))]
+    recs = recs[:len(dens)]  # records [0, L)
+    best = recs[0]
+    for r in recs[1:]:
+        if r[0] > best[0]:
+            best = r
+    return {"max_density": best[0], "res_k": best[1], "res_lmd": best[2]}

@@ -18,10 +18,14 @@
 //                       (CoreParallel/FCSync*.cpp); cell (k, lmd, length) list
 //         wds           FCPathLevelLeft::Execute(mg, tree, true, beta) then
 //                       FCTree::WeightDenestSubgraph (main.cpp:402-424)
+//         wdscoreindex  get_wds per lmd (CoreIndexParallel/CoreIndex.cpp:347-413),
+//                       then CoreIndex::WdsCoreIndex itself (:415-479), whose
+//                       printed answer goes to stdout
 // outputs  <out_prefix>.labels  uint64[n]  id2vtx (MultilayerGraph::LoadId2VtxMap)
 //          <out_prefix>.index   int32[L][n] (index modes)
 //          <out_prefix>.cells   text "lmd k length [density sel]" per cell
 //          <out_prefix>.wds     text "max_density res_k res_lmd res_len" (wds)
+//          <out_prefix>.wdsci   text "lmd density res_k res_lmd" per lmd (wdscoreindex)
 #include "Graph/MultilayerGraph.h"
 #include "Core/FCTree.h"
 #include "Core/FCTreeBuilderLeft.h"
@@ -39,6 +43,16 @@
 
 // Defined with external linkage in CoreIndexParallel/CoreIndex.cpp:62.
 uint* get_index2(MultilayerGraph& mg, uint** degs, uint lmd, ll_uint* id2vtx);
+
+// The per-lmd result record of CoreIndex.cpp:9-13 (declared there, in the
+// .cpp only), needed to call get_wds (CoreIndex.cpp:347, external linkage).
+struct MaxDensity {
+    uint resk;
+    uint reslmd;
+    float density;
+};
+uint* get_wds(MultilayerGraph& mg, uint** degs, uint lmd, ll_uint* id2vtx, MaxDensity& maximum_density, float beta,
+              uint& res_k, uint& res_lmd);
 
 namespace {
 
@@ -154,6 +168,25 @@ int main(int argc, char** argv) {
         if (!f) return 3;
         std::fprintf(f, "%.9g %u %u %u\n", double(maximum_density), res_k, res_lmd, res_len);
         std::fclose(f);
+    } else if (mode == "wdscoreindex") {
+        FILE* f = std::fopen((out + ".wdsci").c_str(), "w");
+        if (!f) return 3;
+        for (uint lmd = 1; lmd <= L; ++lmd) {
+            uint** degs = new uint*[n];
+            for (uint v = 0; v < n; ++v) {
+                degs[v] = new uint[L];
+                for (uint l = 0; l < L; ++l) degs[v][l] = mg.GetGraph(l).GetAdjLst()[v][0];
+            }
+            MaxDensity md{0, 0, 0.0f};
+            uint rk = 0, rl = 0;
+            uint* core = get_wds(mg, degs, lmd, id2vtx.data(), md, float(beta), rk, rl);
+            std::fprintf(f, "%u %.9g %u %u\n", lmd, double(md.density), md.resk, md.reslmd);
+            delete[] core;
+            for (uint v = 0; v < n; ++v) delete[] degs[v];
+            delete[] degs;
+        }
+        std::fclose(f);
+        CoreIndex::WdsCoreIndex(mg, id2vtx.data(), float(beta));
     } else {
         std::fprintf(stderr, "unknown mode %s\n", mode.c_str());
         return 2;

@@ -532,6 +532,90 @@ int densest_impl(hg_graph* g, double beta, hg_wds* out) {
     return HG_OK;
 }
 
+
+// CoreIndex::WdsCoreIndex (CoreIndexParallel/CoreIndex.cpp:307-478) on the
+// device's edge histograms.  Per cell S = S(k, lmd), k = 1..kmax, and layer
+// l the reference increments a float once per directed entry inside S: the
+// count is exact below 2^24 and stays at 2^24 from there (adding 1 to 2^24
+// rounds back to 2^24), so n_edge = min(count, 2^24).  degree = n_edge / 2,
+// sorted descending; density = (float)((degree / (float)|S|) * pow(l+1,
+// (float)beta)); per lmd the record takes (density, k, lmd) on >= from
+// {0, 0, 0}.  The printed answer: records [0, L) (record 0 never written,
+// taken as zeros; record lmd = get_wds's for lmd) sorted by density,
+// descending, stably for L <= 16; the first holds the answer, so lmd = L
+// takes no part (CoreIndex.cpp:474).
+int densest_coreindex_impl(hg_graph* g, double beta, hg_wds* answer, hg_wds* per_lmd) {
+    HIP_TRY(hipSetDevice(g->device));
+    const uint32_t L = g->L;
+    const float fbeta = float(beta);
+    std::vector<double> pw(L);
+    for (uint32_t l = 0; l < L; ++l) pw[l] = std::pow(double(l + 1), double(fbeta));
+    if (!g->hist) {
+        hipError_t e = dalloc(&g->hist, size_t(L) * g->kcap);
+        if (e != hipSuccess) return fail(HG_ERR_NOMEM, "densest histogram allocation failed");
+    }
+    const uint32_t grid = uint32_t(std::min<uint64_t>((g->nchunks * kChunk / 64 + kHistBlock - 1) / kHistBlock,
+                                                       uint64_t(g->cus) * 8));
+    std::vector<hg_wds> rec(size_t(L) + 1);
+    for (auto& r : rec) std::memset(&r, 0, sizeof r);
+    std::vector<uint64_t> h;
+    for (uint32_t i = 0; i < L; ++i) {
+        const int32_t km = g->h_kmax[i];
+        hg_wds& r = rec[size_t(i) + 1];
+        if (km < 1) continue;
+        const uint32_t kc = uint32_t(km) + 1;
+        HIP_TRY(hipMemsetAsync(g->hist, 0, sizeof(uint64_t) * size_t(L) * kc, g->stream));
+        hipLaunchKernelGGL(k_edge_hist, dim3(std::max(grid, 1u)), dim3(kHistBlock), 0, g->stream, g->insts[i], g->chunks,
+                           g->nchunks, g->hist, kc);
+        HIP_TRY(hipGetLastError());
+        h.assign(size_t(L) * kc, 0);
+        HIP_TRY(hipMemcpyAsync(h.data(), g->hist, sizeof(uint64_t) * h.size(), hipMemcpyDeviceToHost, g->stream));
+        HIP_TRY(hipStreamSynchronize(g->stream));
+        const std::vector<int64_t> len = cell_lengths(g, i);
+        std::vector<uint64_t> acc(L, 0);
+        std::vector<float> dens(size_t(km) + 1, 0.0f);
+        std::vector<int32_t> sel(size_t(km) + 1, 0);
+        for (int32_t k = km; k >= 1; --k) {  // suffix sums over m >= k
+            float deg[64];
+            for (uint32_t l = 0; l < L; ++l) {
+                acc[l] += h[size_t(l) * kc + size_t(k)];
+                const float n_edge = acc[l] >= (1ull << 24) ? 16777216.0f : float(acc[l]);
+                deg[l] = n_edge / 2;
+            }
+            std::sort(deg, deg + L, [](float a, float c) { return a > c; });
+            const uint32_t n_node = uint32_t(len[size_t(k)]);
+            float best = 0.0f;
+            for (uint32_t l = 0; l < L; ++l) {
+                const float mad = deg[l] / float(n_node);
+                const float d = float(double(mad) * pw[l]);
+                if (d >= best) {
+                    best = d;
+                    sel[size_t(k)] = int32_t(l + 1);
+                }
+            }
+            dens[size_t(k)] = best;
+        }
+        for (int32_t k = 1; k <= km; ++k) {  // k ascending, >=, as get_wds's loop
+            if (dens[size_t(k)] >= r.max_density) {
+                r.max_density = dens[size_t(k)];
+                r.res_k = uint32_t(k);
+                r.res_lmd = i + 1;
+                r.res_len = uint32_t(len[size_t(k)]);
+                r.num_select_layer = sel[size_t(k)];
+            }
+        }
+    }
+    if (per_lmd)
+        for (uint32_t i = 0; i < L; ++i) per_lmd[i] = rec[size_t(i) + 1];
+    if (answer) {
+        hg_wds best = rec[0];
+        for (uint32_t i = 1; i < L; ++i)  // records [0, L)
+            if (rec[i].max_density > best.max_density) best = rec[i];
+        *answer = best;
+    }
+    return HG_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -630,6 +714,17 @@ int hg_densest(hg_graph* g, double beta, hg_wds* out) {
         return densest_impl(g, beta, out);
     } catch (const std::bad_alloc&) {
         return fail(HG_ERR_NOMEM, "host allocation failed in hg_densest");
+    }
+}
+
+int hg_densest_coreindex(hg_graph* g, double beta, hg_wds* answer, hg_wds* per_lmd) {
+    if (!g) return fail(HG_ERR_ARG, "graph is NULL");
+    if (!g->decomposed) return fail(HG_ERR_STATE, "hg_decompose has not completed");
+    if (!std::isfinite(beta)) return fail(HG_ERR_ARG, "beta must be finite");
+    try {
+        return densest_coreindex_impl(g, beta, answer, per_lmd);
+    } catch (const std::bad_alloc&) {
+        return fail(HG_ERR_NOMEM, "host allocation failed in hg_densest_coreindex");
     }
 }
 

@@ -347,6 +347,7 @@ __device__ __forceinline__ void scan_emit(const Inst& in, InstState* st, Control
 // layers; only those are pushed (their neighbours counted them).
 constexpr uint32_t kPullMaxL = 16;
 constexpr uint64_t kPullHeavy = 1024;  // vertices with more entries are pulled by a wavefront
+constexpr uint32_t kStageWords = 32768;  // level-0 row staging: instances * layers * kBlock <= this
 
 // c[j] += #{entries w of seg : z(w) > j}, j < kPullMaxL (segz = the
 // entries' target z, streamed sequentially).
@@ -394,6 +395,12 @@ __global__ void __launch_bounds__(kBlock) k_init(const Inst* insts, InstState* s
                                                  const uint32_t* deg0, const uint8_t* z, uint32_t n, uint32_t L,
                                                  uint32_t cap, uint32_t H, uint32_t pull) {
     __shared__ ScanShared sh[64];
+    // Pulled rows of light vertices are staged here (16 bits suffice: a light
+    // vertex has at most kPullHeavy entries) and written out as whole runs of
+    // 64 rows, so every line of deg is written once, in full.
+    __shared__ uint16_t s_stage[kStageWords];   // [instance][candidate][layer]
+    __shared__ uint8_t s_c1[kPullMaxL * kBlock];  // layers with a pulled count >= 1
+    const bool stage = pull && I * L * uint32_t(kBlock) <= kStageWords;
     for (uint32_t i = threadIdx.x; i < I; i += blockDim.x) {
         sh[i].keep = 0;
         sh[i].removed = 0;
@@ -420,6 +427,9 @@ __global__ void __launch_bounds__(kBlock) k_init(const Inst* insts, InstState* s
                 }
             } else if (v >= H) {
                 uint64_t pos = uint64_t(in0.off[v]);
+                uint32_t c1v[kPullMaxL];
+#pragma unroll
+                for (uint32_t j = 0; j < kPullMaxL; ++j) c1v[j] = 0;
                 for (uint32_t l = 0; l < L; ++l) {
                     const uint32_t d = deg0[uint64_t(v) * L + l];
                     uint32_t c[kPullMaxL];
@@ -428,8 +438,35 @@ __global__ void __launch_bounds__(kBlock) k_init(const Inst* insts, InstState* s
                     pull_count(in0.adjz + pos, 0, d, 1, c);
                     pos += d;
 #pragma unroll
+                    for (uint32_t j = 0; j < kPullMaxL; ++j) {
+                        if (j >= I) continue;
+                        c1v[j] += c[j] >= 1u;
+                        if (stage) s_stage[(j * kBlock + threadIdx.x) * L + l] = uint16_t(j < zv ? c[j] : 0u);
+                        else if (j < zv) insts[j].deg[uint64_t(v) * L + l] = c[j];
+                    }
+                }
+                if (stage) {
+#pragma unroll
                     for (uint32_t j = 0; j < kPullMaxL; ++j)
-                        if (j < I && j < zv) insts[j].deg[uint64_t(v) * L + l] = c[j];
+                        if (j < I) s_c1[j * kBlock + threadIdx.x] = uint8_t(j < zv ? c1v[j] : 0u);
+                }
+            }
+        }
+        if (stage && !(valid && zv > 0 && v >= H))  // rows without pulled counts are written as zeros
+            for (uint32_t t = 0; t < I * L; ++t) s_stage[((t / L) * kBlock + threadIdx.x) * L + t % L] = 0;
+        if (stage) {
+            // the wavefront's 64 candidates are one run of consecutive ids:
+            // its rows are 64 * L consecutive words of each instance's deg
+            __syncthreads();
+            const uint32_t w0 = threadIdx.x & ~uint32_t(kWave - 1);  // first candidate of this wavefront
+            const uint64_t run0 = (uint64_t((base + w0) / kRun) * G + blockIdx.x) * kRun;
+            if (base + w0 < myruns * kRun && run0 < n) {
+                for (uint32_t j = 0; j < I; ++j) {
+                    uint32_t* rows = insts[j].deg + run0 * L;
+                    for (uint32_t t = lane_id(); t < kRun * L; t += kWave) {
+                        const uint64_t vt = run0 + t / L;
+                        if (vt < n && vt >= H) rows[t] = s_stage[(j * kBlock + w0 + t / L) * L + t % L];
+                    }
                 }
             }
         }
@@ -443,7 +480,9 @@ __global__ void __launch_bounds__(kBlock) k_init(const Inst* insts, InstState* s
             // layers with degree >= 1 among the rows just built
             uint32_t c1 = 0;
             if (valid && zv >= in.lmd) {
-                if (pull) {
+                if (pull && stage && v >= H) {
+                    c1 = s_c1[i * kBlock + threadIdx.x];
+                } else if (pull) {
                     const uint32_t* d = rec_of(in, v);
                     for (uint32_t l = 0; l < L; ++l) c1 += d[l] >= 1u;
                 } else {

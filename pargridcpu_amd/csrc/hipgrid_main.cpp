@@ -11,6 +11,8 @@
 //   HipGridMem  decomposition plus k-shells per cell; replaces -m CoreMem
 //               (main.cpp:331-345).
 //   HipGridWds  weighted densest subgraph; replaces -m WdsList
+//   HipGridWdsCoreIndex  the same cells scored as -m WdsCoreIndex does
+//               (CoreIndexParallel/CoreIndex.cpp:307-478, main.cpp:444-451)
 //               (main.cpp:402-442).
 // Output lines follow the reference's (method/dataset line, layer order,
 // statistics, elapsed time, "mem = .. MB").
@@ -54,8 +56,9 @@ int main(int argc, char** argv) {
         else if (a == "-beta" && has) beta = std::atof(argv[++i]);
         else if (a == "-root" && has) root = argv[++i];
     }
-    if (method != "HipGrid" && method != "HipGridMem" && method != "HipGridWds") {
-        std::fprintf(stderr, "unknown method %s (HipGrid, HipGridMem, HipGridWds)\n", method.c_str());
+    if (method != "HipGrid" && method != "HipGridMem" && method != "HipGridWds" && method != "HipGridWdsCoreIndex") {
+        std::fprintf(stderr, "unknown method %s (HipGrid, HipGridMem, HipGridWds, HipGridWdsCoreIndex)\n",
+                     method.c_str());
         return 2;
     }
     std::printf("method = %s dataset = %s\n", method.c_str(), dataset.c_str());
@@ -121,6 +124,16 @@ int main(int argc, char** argv) {
         std::printf("res_lmd = %u\n", w.res_lmd);
         std::printf("res_len = %u\n", w.res_len);
         std::printf("Wds device time: %g ms\n", tm.densest_ms);
+    }
+    if (method == "HipGridWdsCoreIndex") {
+        hg_wds w{};
+        const auto t0 = std::chrono::steady_clock::now();
+        if (hg_densest_coreindex(eng, beta, &w, nullptr) != HG_OK) return die("hg_densest_coreindex", hg_last_error());
+        const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        std::printf("maximum_density = %g\n", double(w.max_density));
+        std::printf("res_k = %u\n", w.res_k);
+        std::printf("res_lmd = %u\n", w.res_lmd);
+        std::printf("Wds CoreIndex Time = %g\n", secs);
     }
     if (k != 0 && lmd != 0 && method == "HipGrid") {
         std::vector<int32_t> ci(size_t(n) * size_t(L));

@@ -96,6 +96,7 @@ SIGNATURES = {
     "hg_cells": ([_P, C.POINTER(C.c_int64), C.c_int32, C.POINTER(C.c_int32)], C.c_int),
     "hg_shells": ([_P, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_uint32)], C.c_int),
     "hg_densest": ([_P, C.c_double, C.POINTER(Wds)], C.c_int),
+    "hg_densest_coreindex": ([_P, C.c_double, C.POINTER(Wds), C.POINTER(Wds)], C.c_int),
     "hg_cell_density": ([_P, C.POINTER(C.c_double), C.POINTER(C.c_int32), C.c_int32], C.c_int),
     "hg_densest_vertices": ([_P, C.POINTER(C.c_uint32), C.c_int64, C.POINTER(C.c_int64)], C.c_int),
     "hg_last_timing": ([_P, C.POINTER(Timing)], C.c_int),
@@ -224,6 +225,13 @@ class Engine:
         w = Wds()
         _check(lib().hg_densest(self._h, float(beta), C.byref(w)))
         return w
+
+    def densest_coreindex(self, beta: float = 1.0):
+        """-m WdsCoreIndex: (printed answer, best cell per lmd)."""
+        ans = Wds()
+        per = (Wds * self.L)()
+        _check(lib().hg_densest_coreindex(self._h, float(beta), C.byref(ans), per))
+        return ans, [per[i] for i in range(self.L)]
 
     def cell_density(self, cap: int):
         d = np.zeros((self.L, cap), dtype=np.float64)

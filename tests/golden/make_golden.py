@@ -18,7 +18,16 @@ the CSR) and the reference's outputs are stored:
   lpe_count, lpe_edge_fc      lpe's printed `count` and num_of_edge_fc lines
   labels       uint64 [n_ref] the reference's id -> label map (loader case)
 
-Usage: python tests/golden/make_golden.py   (after __graft_entry__.build())
+and, per graph, <name>_wdsci.npz from CoreIndex::WdsCoreIndex
+(CoreIndexParallel/CoreIndex.cpp:307-478):
+  ci_<beta>          float32 [L] get_wds's density per lmd (record values)
+  ci_<beta>_res      int64 [L][2] its res_k, res_lmd
+  ci_<beta>_printed  the printed answer: density (6 significant digits),
+                     res_k, res_lmd
+
+Usage: python tests/golden/make_golden.py [--wdsci-only]
+
+Run after __graft_entry__.build().
 """
 from __future__ import annotations
 
@@ -158,11 +167,37 @@ def make_loader_fixture(workdir: str) -> dict:
             "core_index": idx}
 
 
+def make_wdsci_fixture(name: str, spec, workdir: str) -> dict:
+    g = MultilayerGraph.generate(spec)
+    ds = os.path.join(workdir, name)
+    if not os.path.exists(ds):
+        g.write_text(ds)
+    out = {"name": name, "checksum": graph_checksum(g)}
+    for beta in BETAS:
+        pre = os.path.join(workdir, f"{name}_wdsci_{beta}")
+        stdout = run_harness("wdscoreindex", ds, pre, beta)
+        rows = [line.split() for line in open(pre + ".wdsci")]
+        tag = "ci_" + str(beta).replace(".", "p")
+        out[tag] = np.array([float(r[1]) for r in rows], dtype=np.float32)
+        out[tag + "_res"] = np.array([[int(r[2]), int(r[3])] for r in rows], dtype=np.int64)
+        md = float(re.search(r"^maximum_density = (\S+)", stdout, re.M).group(1))
+        rk = int(re.search(r"^res_k = (\d+)", stdout, re.M).group(1))
+        rl = int(re.search(r"^res_lmd = (\d+)", stdout, re.M).group(1))
+        out[tag + "_printed"] = np.array([md, rk, rl], dtype=np.float64)
+    return out
+
+
 def main() -> int:
     if not os.path.exists(HARNESS):
         print("oracle/_ref/ref_harness is not built (make -C oracle ref)", file=sys.stderr)
         return 2
     with tempfile.TemporaryDirectory(prefix="hg_golden_") as work:
+        for name, spec in GRAPHS.items():
+            fx = make_wdsci_fixture(name, spec, work)
+            np.savez_compressed(os.path.join(HERE, f"{name}_wdsci.npz"), **fx)
+            print(f"wrote {name}_wdsci: {fx['ci_1p0'].tolist()} printed {fx['ci_1p0_printed'].tolist()}")
+        if "--wdsci-only" in sys.argv:
+            return 0
         for name, spec in GRAPHS.items():
             fx = make_graph_fixture(name, spec, work)
             np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **fx)

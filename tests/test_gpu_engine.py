@@ -181,3 +181,24 @@ def test_errors():
         assert e.value.code == -5
         with pytest.raises(HgError):
             eng.densest(1.0)
+
+
+def test_wds_coreindex_float_counter_saturation():
+    """A 4200-clique in two identical layers: every cell is the whole clique
+    with 4200 * 4199 > 2^24 directed entries per layer, so the reference's
+    float edge counter stops at 2^24 (CoreIndexParallel/CoreIndex.cpp:317-324)."""
+    n = 4200
+    es = G.clique(range(n))
+    mg = G.graph(n, [es, es])
+    say("case wdscoreindex saturation (4200-clique, 2 layers)")
+    with Engine.from_graph(mg) as eng:
+        eng.decompose()
+        ci = eng.core_index()
+        assert (ci == n - 1).all()
+        ans, per = eng.densest_coreindex(1.0)
+    mad = np.float32(np.float32(1 << 24) / np.float32(2)) / np.float32(n)
+    want = np.float32(float(mad) * 2.0)  # the second-ranked layer, pow(2, 1)
+    assert [np.float32(w.max_density) for w in per] == [want, want]
+    assert [(w.res_k, w.res_lmd) for w in per] == [(n - 1, 1), (n - 1, 2)]
+    assert (np.float32(ans.max_density), ans.res_k, ans.res_lmd) == (want, n - 1, 1)  # lmd = L takes no part
+

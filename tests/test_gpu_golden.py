@@ -53,6 +53,28 @@ def test_engine_matches_reference(name):
         assert np.array_equal(verts, np.nonzero(fx["core_index"][res[1] - 1] >= res[0])[0].astype(np.uint32))
 
 
+@pytest.mark.parametrize("name", sorted(GRAPHS))
+def test_wds_coreindex_matches_reference(name):
+    """-m WdsCoreIndex: per-lmd records of get_wds and the printed answer."""
+    fx = fixture(name + "_wdsci")
+    g = MultilayerGraph.generate(GRAPHS[name])
+    assert graph_checksum(g) == str(fx["checksum"])
+    say(f"case wdscoreindex {name}")
+    with Engine.from_graph(g) as eng:
+        eng.decompose()
+        for beta in (1.0, 0.5, 2.0):
+            tag = "ci_" + str(beta).replace(".", "p")
+            ans, per = eng.densest_coreindex(beta)
+            assert [np.float32(w.max_density) for w in per] == fx[tag].tolist()
+            assert [[w.res_k, w.res_lmd] for w in per] == fx[tag + "_res"].tolist()
+            ref = O.wds_coreindex(g.n, g.layers(), beta)["answer"]
+            assert (np.float32(ans.max_density), ans.res_k, ans.res_lmd) == (ref["max_density"], ref["res_k"],
+                                                                              ref["res_lmd"])
+            md, rk, rl = fx[tag + "_printed"]
+            if g.L > 1:  # with L = 1 the reference prints an uninitialised record
+                assert float(f"{float(ans.max_density):g}") == md and (ans.res_k, ans.res_lmd) == (int(rk), int(rl))
+
+
 def test_loader_case(tmp_path):
     fx = fixture("loader")
     d = tmp_path / "loader"
@@ -87,6 +109,23 @@ def test_driver_wds_c1(tmp_path):
     md = float(re.search(r"maximum_density = ([0-9.e+-]+)", out).group(1))
     assert abs(md - float(fx["wds_1p0"][0])) <= 1e-4 * float(fx["wds_1p0"][0])  # printed with %g
     assert "mem = " in out and "HipGrid Elapsed time:" in out
+
+
+def test_driver_wds_coreindex_c1(tmp_path):
+    fx = fixture("c1_wdsci")
+    g = MultilayerGraph.generate(GRAPHS["c1"])
+    root = tmp_path / "dataset"
+    g.write_text(str(root / "c1"))
+    exe = os.path.join(ROOT, "pargridcpu_amd", "bin", "hipgrid")
+    say("case driver HipGridWdsCoreIndex c1")
+    r = subprocess.run([exe, "-d", "c1", "-m", "HipGridWdsCoreIndex", "-root", str(root)], capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    md, rk, rl = fx["ci_1p0_printed"]
+    assert float(re.search(r"maximum_density = (\S+)", r.stdout).group(1)) == md
+    assert int(re.search(r"res_k = (\d+)", r.stdout).group(1)) == int(rk)
+    assert int(re.search(r"res_lmd = (\d+)", r.stdout).group(1)) == int(rl)
+    assert "Wds CoreIndex Time = " in r.stdout
 
 
 def test_int32_offsets_and_layer_order():

@@ -169,6 +169,66 @@ def test_wds_independent(name, beta):
     assert (w["res_k"], w["res_lmd"], w["res_len"]) == res
 
 
+@pytest.mark.parametrize("name", sorted(GRAPHS))
+@pytest.mark.parametrize("beta", [1.0, 0.5, 2.0])
+def test_wds_coreindex_matches_reference(name, beta):
+    """WdsCoreIndex restatement against get_wds's per-lmd records and the
+    answer CoreIndex::WdsCoreIndex printed (reference built from source)."""
+    g, _ = fixture_graph(name)
+    fx = load_fixture(name + "_wdsci")
+    tag = "ci_" + str(beta).replace(".", "p")
+    r = O.wds_coreindex(g.n, g.layers(), beta)
+    assert np.array_equal(r["density"], fx[tag])
+    assert np.array_equal(np.stack([r["res_k"], r["res_lmd"]], axis=1).astype(np.int64), fx[tag + "_res"])
+    md, rk, rl = fx[tag + "_printed"]
+    if g.L > 1:  # with L = 1 the reference prints an uninitialised record
+        a = r["answer"]
+        assert float(f"{float(a['max_density']):g}") == md
+        assert (a["res_k"], a["res_lmd"]) == (int(rk), int(rl))
+
+
+def wds_coreindex_numpy(g, ci, beta):
+    """Independent restatement of get_wds's cell scoring (numpy, float32)."""
+    best = [(np.float32(0.0), 0, 0) for _ in range(g.L)]
+    lay = g.layers()
+    fb = np.float32(beta)
+    for lmd in range(1, g.L + 1):
+        for k in range(1, int(ci[lmd - 1].max()) + 1):
+            inside = ci[lmd - 1] >= k
+            n_node = int(inside.sum())
+            degs = []
+            for lyr in lay:
+                src = np.repeat(np.arange(g.n), np.diff(lyr.row_ptr))
+                cnt = int(np.count_nonzero(inside[src] & inside[lyr.col.astype(np.int64)]))
+                degs.append(np.float32(min(cnt, 1 << 24)) / np.float32(2))
+            degs.sort(reverse=True)
+            for i, d in enumerate(degs):
+                mad = np.float32(d / np.float32(n_node))
+                dens = np.float32(float(mad) * math.pow(i + 1, float(fb)))
+                if dens >= best[lmd - 1][0]:
+                    best[lmd - 1] = (dens, k, lmd)
+    return best
+
+
+@pytest.mark.parametrize("name", ["g300_L3", "g2000_L5", "g1500_L8"])
+def test_wds_coreindex_independent(name):
+    g, fx = fixture_graph(name)
+    r = O.wds_coreindex(g.n, g.layers(), 0.5)
+    for i, (d, k, l) in enumerate(wds_coreindex_numpy(g, fx["core_index"], 0.5)):
+        assert (r["density"][i], r["res_k"][i], r["res_lmd"][i]) == (d, k, l)
+
+
+def test_float_counter_saturates_at_2_pow_24():
+    """A float incremented by 1 is exact up to 2^24 and stays there: the
+    reference's per-layer edge counter equals min(count, 2^24)."""
+    f = np.float32(1 << 24) - np.float32(3)
+    seq = []
+    for _ in range(8):
+        f = np.float32(f + np.float32(1))
+        seq.append(int(f))
+    assert seq == [16777214, 16777215, 16777216, 16777216, 16777216, 16777216, 16777216, 16777216]
+
+
 def test_planted_mistake_is_caught():
     """A mistake planted in a copy of the oracle's result (a dropped vertex
     in one cell) must fail the reference comparison."""
