@@ -122,7 +122,7 @@ def run_batched(steps: int, count: int = 256) -> dict:
     D = sum(g.directed_entries() for g in gs)
     log(f"[bench] batched: generated {count} graphs (D={D}) in {time.perf_counter() - t0:.1f}s")
     b = Batch(gs)
-    best, tuned_ms = b.tune(reps=3)
+    best, tuned_ms = b.tune(reps=1)
     log(f"[bench] batched: autotuner picked {best} ({tuned_ms:.3f} ms)")
     walls = []
     for _ in range(max(steps, 3)):
@@ -151,7 +151,7 @@ def run_batched(steps: int, count: int = 256) -> dict:
             "exact": bool(exact), "params": best}
 
 
-def run_serving(n_requests: int = 120, rate: float = 8.0) -> dict:
+def run_serving(n_requests: int = 160, rate: float = 20.0) -> dict:
     """Serving form: Poisson arrivals at `rate` requests/s, 80% decompositions
     and 20% densest-subgraph searches; graphs from the batched workload's
     distribution, 10% of them large (n = 5M, L = 12, 60M sampled edges);
