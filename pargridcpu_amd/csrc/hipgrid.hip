@@ -329,6 +329,8 @@ int create_impl(const hg_layer_csr* layers, int32_t L, int64_t n, int32_t offset
         in.cap = g->cap;
         in.lmagic = g->L == 1 ? 0xFFFFFFFFu : uint32_t((1ull << 32) / g->L);  // div_L needs magic < 2^32
         in.flags = env_int("HG_AGG", 1) == 2 ? 0u : kFlagAggregate;  // HG_AGG=2 disables combining
+        in.heavy = uint32_t(env_int("HG_HEAVY", int(kHeavy)));
+        in.pad = 0;
     }
     return HG_OK;
 }
@@ -346,11 +348,11 @@ const char* err_name(int32_t e) {
     }
 }
 
-// Launches step s: k_advance, then the scan (level-0 init at s = 0), then
-// the cascade.  Kernels of instances that are done return at once.
+// Launches step s: the scan (level-0 init at s = 0), which also decides
+// the step, then the cascade.  Kernels of instances that are done return at
+// once.
 void launch_step(hg_graph* g, uint32_t s) {
     const uint32_t L = g->L;
-    hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, g->stream, g->d_states, g->d_ctl, L, s);
     if (s == 0) {
         if (g->pull && g->H > 0) {
             const uint32_t gh = uint32_t(std::min<uint64_t>((uint64_t(g->H) * kWave + kBlock - 1) / kBlock,
@@ -374,11 +376,14 @@ int decompose_impl(hg_graph* g) {
     std::vector<InstState> st(L);
     std::memset(st.data(), 0, sizeof(InstState) * L);
     for (uint32_t i = 0; i < L; ++i) {
+        // step 0 decides from the slots of parity 1: level -1 + 1 = 0
         st[i].alive[0] = g->n;
+        st[i].dphase[1] = kPhaseScan;
+        st[i].dk[1] = -1;
         if (i + 1 > g->max_nz) {
             // No vertex has edges in lmd layers: S(1, lmd) is empty, every
             // index is 0 (k_init writes the zeros).
-            st[i].phase = kPhaseDone;
+            st[i].dphase[1] = kPhaseDone;
             st[i].alive[0] = 0;
         }
     }
@@ -428,7 +433,7 @@ int decompose_impl(hg_graph* g) {
     g->timing.scan_ms = -1;
     g->timing.cascade_ms = -1;
     g->timing.levels = int64_t(s);
-    g->timing.launches = int64_t(s) * 3 + 1;
+    g->timing.launches = int64_t(s) * 2 + 3;
 
     g->h_count.assign(size_t(g->kcap) * L, 0);
     HIP_TRY(hipMemcpyAsync(g->h_count.data(), g->d_ihist, sizeof(int64_t) * g->h_count.size(), hipMemcpyDeviceToHost,

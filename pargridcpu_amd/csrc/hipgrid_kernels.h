@@ -38,7 +38,7 @@
 //     cascade strides that list over all blocks.  Vertices removed during a
 //     cascade go to the discovering block's own FIFO ring (slots reserved
 //     with LDS atomics) and are expanded by that block; heavy ones
-//     (> kHeavy entries) and ring overflow go to the next step's list, so a
+//     (> heavy entries, default kHeavy) and ring overflow go to the next step's list, so a
 //     hub is spread over the whole GPU.  Nothing ever spins.
 //   * Liveness is read from a per-instance bitmap (n bits: 100 MB for all
 //     eight instances of a 100M-vertex graph, so it stays in the 256 MB
@@ -46,9 +46,10 @@
 //     cached bit, not a random HBM line.  The bit is cleared when the
 //     vertex is removed; a stale bit only costs a harmless decrement of a
 //     removed vertex (its cnt is already below lmd).
-//   * One step = k_advance (1 block: per-instance state machine) + A (scan,
-//     or level-0 init) + B (cascade).  A step either starts a new level or
-//     continues the current one while deferred chunks remain.
+//   * One step = A (scan, or level-0 init; every block first decides, per
+//     instance, whether the step starts a new level, continues the current
+//     one while deferred chunks remain, or finds the instance done) + B
+//     (cascade).
 //
 // Adjacency is vertex-major: entries of all layers of v are contiguous at
 // adj[off[v] .. off[v+1]); an entry is u*L + l, which is also the index of
@@ -67,7 +68,7 @@ constexpr int kWave = 64;
 constexpr int kBlock = 512;  // threads per block of the level kernels (8 wavefronts)
 constexpr int kWavesPerBlock = kBlock / kWave;
 constexpr int kChunk = 64;         // static chunk list granularity (densest pass)
-constexpr uint32_t kHeavy = 4096;  // vertices with more entries are deferred
+constexpr uint32_t kHeavy = 4096;  // default: vertices with more entries are deferred
 constexpr uint32_t kDefer = 4096;  // entries per deferred chunk
 constexpr uint32_t kRun = 64;      // consecutive vertex ids per ownership run
 
@@ -91,6 +92,8 @@ struct Inst {
     uint32_t n, L, lmd, cap;
     uint32_t lmagic;  // min(floor(2^32 / L), 2^32 - 1)
     uint32_t flags;   // kFlagAggregate: combine decrements in LDS
+    uint32_t heavy;   // cascade removals with more entries are deferred to the next step
+    uint32_t pad;
 };
 
 __device__ __forceinline__ uint32_t* rec_of(const Inst& in, uint32_t v) { return in.deg + uint64_t(v) * in.L; }
@@ -103,17 +106,23 @@ __device__ __forceinline__ void clear_alive(const Inst& in, uint32_t v) {
 constexpr uint32_t kFlagAggregate = 1;
 
 // Per-instance state; the atomically updated words sit on lines of their own.
+// Step s uses the slots of parity s & 1: its phase and level (written once,
+// by block 0 of the step's level kernel), the chunks its scan pushed
+// (pfront) and the chunks the previous step's cascade deferred to it
+// (pdefer).  One chunk list per parity holds both kinds: a step either
+// continues with deferred chunks or scans, never both.
 struct alignas(128) InstState {
-    int32_t phase;
-    int32_t k;
-    int32_t pad0[30];
-    unsigned long long pcount[2][16];  // deferred chunk counts (one line each)
+    int32_t dphase[2];
+    int32_t dk[2];
+    int32_t pad0[28];
+    unsigned long long pdefer[2][16];  // one line each
+    unsigned long long pfront[2][16];
     unsigned long long alive[16];      // vertices not yet removed (decremented mod 2^64)
 };
 
 struct alignas(128) Control {
     int32_t error;   // nonzero: device-side check failed
-    int32_t active;  // instances not done after the last k_advance
+    int32_t active;  // instances not done at the last level kernel
     int32_t pad[30];
 };
 
@@ -282,33 +291,54 @@ __global__ void k_build_chunks(const int64_t* off, const uint64_t* chunk_off, ui
 }
 
 // ------------------------------------------------------------ step control
-// One thread per instance.  s = step number.  Decides what step s does:
-// continue the level while the deferred list of parity s is non-empty,
-// finish when no vertex is alive, otherwise start the next level.
-__global__ void k_advance(InstState* states, Control* ctl, uint32_t I, uint32_t s) {
-    __shared__ int32_t s_active;
-    if (threadIdx.x == 0) s_active = 0;
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < I; i += blockDim.x) {
-        InstState* st = &states[i];
-        if (st->phase == kPhaseDone) continue;
-        if (s == 0) {
-            st->phase = kPhaseScan;
-            st->k = 0;
-        } else if (st->pcount[s & 1][0] > 0) {
-            st->phase = kPhaseCont;
-        } else if (st->alive[0] == 0) {
-            st->phase = kPhaseDone;
-            continue;
-        } else {
-            st->phase = kPhaseScan;
-            st->k += 1;
+// What step s does for an instance, from the previous step's slots: continue
+// the level while the previous cascade deferred chunks to this step, finish
+// when no vertex is alive, otherwise start the next level.  Every block of
+// the step's level kernel (k_init or k_scan) decides for itself: the inputs
+// are not written during the kernel, except the alive count, which only
+// falls -- a block that sees it reach 0 has no alive vertex left in its
+// slice, and once every vertex is removed the rest of the step changes no
+// index.  Block 0 records its decision for the cascade and the next step.
+__device__ __forceinline__ int32_t decide(InstState* st, uint32_t s, int32_t* k) {
+    const uint32_t p = (s + 1u) & 1u;  // parity of step s - 1
+    const int32_t prev = st->dphase[p];
+    *k = st->dk[p];
+    if (prev == kPhaseDone) return kPhaseDone;
+    if (st->pdefer[s & 1u][0] > 0) return kPhaseCont;
+    if (s > 0 && __hip_atomic_load(&st->alive[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) return kPhaseDone;
+    *k += 1;
+    return kPhaseScan;
+}
+
+struct StepShared {
+    int32_t phase[64];
+    int32_t k[64];
+};
+
+// Every thread i < I decides instance i into sh; block 0 records the
+// decisions, clears the next step's counters and reports the active count.
+// All threads call.
+__device__ __forceinline__ void step_decide(InstState* states, Control* ctl, uint32_t I, uint32_t s,
+                                            StepShared* sh) {
+    if (threadIdx.x < I) {
+        int32_t k;
+        const int32_t ph = decide(&states[threadIdx.x], s, &k);
+        sh->phase[threadIdx.x] = ph;
+        sh->k[threadIdx.x] = k;
+        if (blockIdx.x == 0) {
+            InstState* st = &states[threadIdx.x];
+            st->dphase[s & 1u] = ph;
+            st->dk[s & 1u] = k;
+            st->pdefer[(s + 1u) & 1u][0] = 0;
+            st->pfront[(s + 1u) & 1u][0] = 0;
         }
-        st->pcount[(s + 1) & 1][0] = 0;
-        atomicAdd(&s_active, 1);
     }
     __syncthreads();
-    if (threadIdx.x == 0) ctl->active = s_active;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        int32_t active = 0;
+        for (uint32_t i = 0; i < I; ++i) active += sh->phase[i] != kPhaseDone;
+        ctl->active = active;
+    }
 }
 
 // ------------------------------------------------------------------ scans
@@ -333,7 +363,7 @@ __device__ __forceinline__ void scan_emit(const Inst& in, InstState* st, Control
     }
     const uint64_t rm = __ballot(rem);
     if (rm && lane_id() == 0) atomicAdd(&sh->removed, uint32_t(__popcll(rm)));
-    block_defer_push(in, defer_list(in, parity), &st->pcount[parity][0], ctl, rem, v, sh->tmp, &sh->base);
+    block_defer_push(in, defer_list(in, parity), &st->pfront[parity][0], ctl, rem, v, sh->tmp, &sh->base);
 }
 
 // ------------------------------------------------------------- level 0
@@ -395,6 +425,7 @@ __global__ void __launch_bounds__(kBlock) k_init(const Inst* insts, InstState* s
                                                  const uint32_t* deg0, const uint8_t* z, uint32_t n, uint32_t L,
                                                  uint32_t cap, uint32_t H, uint32_t pull) {
     __shared__ ScanShared sh[64];
+    __shared__ StepShared ss;
     // Pulled rows of light vertices are staged here (16 bits suffice: a light
     // vertex has at most kPullHeavy entries) and written out as whole runs of
     // 64 rows, so every line of deg is written once, in full.
@@ -405,6 +436,7 @@ __global__ void __launch_bounds__(kBlock) k_init(const Inst* insts, InstState* s
         sh[i].keep = 0;
         sh[i].removed = 0;
     }
+    step_decide(states, ctl, I, 0u, &ss);
     __syncthreads();
     const uint64_t lo = uint64_t(blockIdx.x) * cap;  // this block's list slice
     const uint32_t G = gridDim.x;
@@ -472,7 +504,7 @@ __global__ void __launch_bounds__(kBlock) k_init(const Inst* insts, InstState* s
         }
         for (uint32_t i = 0; i < I; ++i) {
             const Inst& in = insts[i];
-            if (states[i].phase == kPhaseDone) {  // lmd above every vertex's layer count: all zero
+            if (ss.phase[i] == kPhaseDone) {  // lmd above every vertex's layer count: all zero
                 if (valid) *idx_of(in, v) = 0;
                 continue;
             }
@@ -516,13 +548,13 @@ __global__ void __launch_bounds__(kBlock) k_init(const Inst* insts, InstState* s
                 if (alive)
                     in.list[lo + kb + __builtin_amdgcn_mbcnt_hi(uint32_t(km >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(km), 0u))] = v;
             }
-            block_defer_push(in, defer_list(in, 0u), &states[i].pcount[0][0], ctl, push, v, sh[i].tmp, &sh[i].base);
+            block_defer_push(in, defer_list(in, 0u), &states[i].pfront[0][0], ctl, push, v, sh[i].tmp, &sh[i].base);
         }
     }
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < I; i += blockDim.x) {
         const Inst& in = insts[i];
-        if (states[i].phase == kPhaseDone) continue;
+        if (ss.phase[i] == kPhaseDone) continue;
         in.blk[2 * blockIdx.x] = sh[i].keep;
         if (sh[i].removed) atomicAdd(&states[i].alive[0], 0ull - (unsigned long long)sh[i].removed);
     }
@@ -535,11 +567,13 @@ __global__ void __launch_bounds__(kBlock) k_init(const Inst* insts, InstState* s
 __global__ void __launch_bounds__(kBlock) k_scan(const Inst* insts, InstState* states, Control* ctl, uint32_t I,
                                                  uint32_t s) {
     __shared__ ScanShared sh;
+    __shared__ StepShared ss;
+    step_decide(states, ctl, I, s, &ss);
     for (uint32_t i = 0; i < I; ++i) {
         InstState* st = &states[i];
-        if (st->phase != kPhaseScan) continue;
+        if (ss.phase[i] != kPhaseScan) continue;
         const Inst& in = insts[i];
-        const int32_t k = st->k;
+        const int32_t k = ss.k[i];
         const uint32_t thr = uint32_t(k) + 1u;
         const uint32_t L = in.L;
         const uint64_t lo = uint64_t(blockIdx.x) * in.cap;
@@ -618,7 +652,7 @@ __device__ __forceinline__ void emit_removed(const Inst& in, InstState* st, Cont
     const uint32_t cap = in.cap;
     uint64_t d = 0;
     if (rem) d = uint64_t(in.off[u + 1] - in.off[u]);
-    const bool light = rem && d <= kHeavy;
+    const bool light = rem && d <= in.heavy;
     const uint64_t lm = __ballot(light);
     bool placed = false;
     if (lm) {
@@ -645,7 +679,7 @@ __device__ __forceinline__ void emit_removed(const Inst& in, InstState* st, Cont
             placed = light;
         }
     }
-    defer_push(in, defer_list(in, next_parity), &st->pcount[next_parity][0], ctl, rem && !placed, u);
+    defer_push(in, defer_list(in, next_parity), &st->pdefer[next_parity][0], ctl, rem && !placed, u);
 }
 
 // Applies U decrements deg[e[r]] -= c[r] (where valid[r]) with the cascade
@@ -811,10 +845,10 @@ __global__ void __launch_bounds__(kBlock) k_cascade(const Inst* insts, InstState
     uint64_t seq = 0;  // chunks of the instances before i
     for (uint32_t i = 0; i < I; ++i) {
         InstState* st = &states[i];
-        const int32_t phase = st->phase;
-        if (phase == kPhaseDone) continue;
         const uint32_t par = s & 1u, npar = par ^ 1u;
-        const uint64_t pc = st->pcount[par][0];
+        const int32_t phase = st->dphase[par];
+        if (phase == kPhaseDone) continue;
+        const uint64_t pc = st->pdefer[par][0] + st->pfront[par][0];  // one of them is zero
         if (pc == 0) continue;  // empty frontier: the ring starts empty, nothing to do
         // chunk j of instance i is item seq + j of the sequence; it goes to
         // block (seq + j) mod G, so this block's first chunk is j = first
@@ -822,7 +856,7 @@ __global__ void __launch_bounds__(kBlock) k_cascade(const Inst* insts, InstState
         seq += pc;
         if (first >= pc) continue;
         const Inst& in = insts[i];
-        const int32_t k = st->k;
+        const int32_t k = st->dk[par];
         uint32_t* ring = in.ring + uint64_t(b) * in.cap;
         if (threadIdx.x == 0) {
             sh.head = 0;
