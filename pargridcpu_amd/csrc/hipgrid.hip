@@ -89,6 +89,11 @@ struct hg_graph {
     std::vector<double> h_density;   // [L][kcap]
     std::vector<int32_t> h_sel;
     hg_timing timing{};
+    // host sources of small uploads (kept until the next call)
+    std::vector<InstState> h_states;
+    Control h_ctl_init{};
+    std::vector<double> h_pow;
+    std::vector<int64_t> h_len;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
 };
 
@@ -201,8 +206,10 @@ int create_impl(const hg_layer_csr* layers, int32_t L, int64_t n, int32_t offset
     g->H = H;
     g->pull = uint32_t(L) <= kPullMaxL && env_int("HG_PULL", 1) == 1 ? 1u : 0u;
     g->kcap = uint32_t(maxdeg + 2);
-    // Level-kernel grid: HG_BLOCKS overrides 2 blocks per CU.
-    const uint32_t want_g = uint32_t(env_int("HG_BLOCKS", 2 * g->cus));
+    // Level-kernel grid: 3 blocks per CU (the cascade's LDS allows 3; a sweep
+    // on the 1M- and 100M-vertex workloads picked it over 2 and 4); HG_BLOCKS
+    // overrides.
+    const uint32_t want_g = uint32_t(env_int("HG_BLOCKS", 3 * g->cus));
     g->G = std::max(1u, std::min<uint32_t>(want_g, uint32_t(n)));
     // list slice per block: its runs of kRun ids, dealt round robin
     g->cap = uint32_t(((uint64_t(n) + kRun - 1) / kRun + g->G - 1) / g->G * kRun);
@@ -301,7 +308,11 @@ int create_impl(const hg_layer_csr* layers, int32_t L, int64_t n, int32_t offset
         hipLaunchKernelGGL(k_build_chunks, dim3(grid3), dim3(kBlock), 0, g->stream, g->off, d_coff, g->n, g->chunks);
         e = hipGetLastError();
     }
-    if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
+    // also on failure: queued copies read the host vectors and temporaries freed below
+    {
+        const hipError_t es = hipStreamSynchronize(g->stream);
+        if (e == hipSuccess) e = es;
+    }
     cleanup();
     if (e != hipSuccess)
         return fail(e == hipErrorOutOfMemory ? HG_ERR_NOMEM : HG_ERR_HIP,
@@ -373,7 +384,10 @@ int decompose_impl(hg_graph* g) {
     const uint32_t L = g->L;
     g->decomposed = false;
     g->densest_done = false;
-    std::vector<InstState> st(L);
+    // host sources of the uploads below live in the handle: a queued copy
+    // from pageable memory may read them after this call has returned early
+    std::vector<InstState>& st = g->h_states;
+    st.assign(L, InstState{});
     std::memset(st.data(), 0, sizeof(InstState) * L);
     for (uint32_t i = 0; i < L; ++i) {
         // step 0 decides from the slots of parity 1: level -1 + 1 = 0
@@ -387,7 +401,7 @@ int decompose_impl(hg_graph* g) {
             st[i].alive[0] = 0;
         }
     }
-    Control ctl;
+    Control& ctl = g->h_ctl_init;
     std::memset(&ctl, 0, sizeof ctl);
     ctl.active = int32_t(L);
     HIP_TRY(hipMemcpyAsync(g->d_insts, g->insts.data(), sizeof(Inst) * L, hipMemcpyHostToDevice, g->stream));
@@ -470,7 +484,8 @@ std::vector<int64_t> cell_lengths(const hg_graph* g, uint32_t i) {
 int densest_impl(hg_graph* g, double beta, hg_wds* out) {
     HIP_TRY(hipSetDevice(g->device));
     const uint32_t L = g->L;
-    std::vector<double> pw(L);
+    std::vector<double>& pw = g->h_pow;  // lives in the handle (source of a queued copy)
+    pw.assign(L, 0.0);
     for (uint32_t l = 0; l < L; ++l) pw[l] = std::pow(double(l + 1), beta);
     HIP_TRY(hipMemcpyAsync(g->d_pow, pw.data(), sizeof(double) * L, hipMemcpyHostToDevice, g->stream));
     int32_t kmax_all = 0;
@@ -492,7 +507,8 @@ int densest_impl(hg_graph* g, double beta, hg_wds* out) {
         HIP_TRY(hipMemsetAsync(g->hist, 0, sizeof(uint64_t) * size_t(L) * kc, g->stream));
         hipLaunchKernelGGL(k_edge_hist, dim3(std::max(grid, 1u)), dim3(kHistBlock), 0, g->stream, g->insts[i], g->chunks,
                            g->nchunks, g->hist, kc);
-        std::vector<int64_t> len = cell_lengths(g, i);
+        std::vector<int64_t>& len = g->h_len;  // lives in the handle (source of a queued copy)
+        len = cell_lengths(g, i);
         HIP_TRY(hipMemcpyAsync(g->d_cell_len, len.data(), sizeof(int64_t) * len.size(), hipMemcpyHostToDevice,
                                g->stream));
         const uint32_t cells = uint32_t(km);
@@ -699,7 +715,11 @@ int hg_shells(hg_graph* g, int32_t kcap, int64_t* shell_off, uint32_t* shell_vtx
         const size_t stride = size_t(kcap) + 1;
         std::vector<int64_t> cnt(stride * g->L + 1, 0);
         for (uint32_t i = 0; i < g->L; ++i)
-            for (uint32_t v = 0; v < g->n; ++v) cnt[i * stride + size_t(ci[size_t(i) * g->n + v]) + 1]++;
+            for (uint32_t v = 0; v < g->n; ++v) {
+                const int32_t x = ci[size_t(i) * g->n + v];
+                if (x < 0 || x >= kcap) return fail(HG_ERR_DEVICE, "core index outside [0, kcap) in the result");
+                cnt[i * stride + size_t(x) + 1]++;
+            }
         shell_off[0] = 0;
         for (size_t t = 0; t < stride * g->L; ++t) shell_off[t + 1] = shell_off[t] + cnt[t + 1];
         std::vector<int64_t> pos(shell_off, shell_off + stride * g->L);

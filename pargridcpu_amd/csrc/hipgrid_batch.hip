@@ -82,6 +82,7 @@ struct hg_batch {
     // results
     bool decomposed = false;
     std::vector<int32_t> h_kmax;  // by item id
+    std::vector<BItem> h_queue;   // items in queue order (upload source)
     double decompose_ms = 0, densest_ms = 0;
 };
 
@@ -405,7 +406,11 @@ int create_impl(const hg_batch_graph* gs, int32_t count, void* stream, hg_batch*
                            d_inv + b->vb[g], d_zc + b->vb[g], b->n[g], b->L[g], b->adj + eb[g], b->adjz + eb[g]);
         e = hipGetLastError();
     }
-    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    // also on failure: queued copies read the host vectors and temporaries freed below
+    {
+        const hipError_t es = hipStreamSynchronize(b->stream);
+        if (e == hipSuccess) e = es;
+    }
     cleanup();
     if (e != hipSuccess)
         return fail(e == hipErrorOutOfMemory ? HG_ERR_NOMEM : HG_ERR_HIP, std::string("batch upload: ") + hipGetErrorString(e));
@@ -413,7 +418,8 @@ int create_impl(const hg_batch_graph* gs, int32_t count, void* stream, hg_batch*
 }
 
 int upload_items(hg_batch* b) {
-    std::vector<BItem> q(b->items);
+    std::vector<BItem>& q = b->h_queue;  // lives in the handle (source of a queued copy)
+    q = b->items;
     if (b->params.order == 0) {
         std::vector<size_t> ord(q.size());
         for (size_t i = 0; i < ord.size(); ++i) ord[i] = i;
@@ -541,6 +547,10 @@ int densest_impl(hg_batch* b, const int32_t* graphs, int32_t count, double beta,
     std::sort(sel.begin(), sel.end());
     sel.erase(std::unique(sel.begin(), sel.end()), sel.end());
     HIP_TRY(hipEventRecord(b->ev[0], b->stream));
+    // Host source of the item upload below: an asynchronous copy from
+    // pageable memory may read it when the stream reaches the copy, so it
+    // lives until the stream is synchronised.
+    std::vector<BItem> sitems(sel.size());
     if (!sel.empty()) {
         if (sel.size() > b->sel_cap) {
             if (b->d_sel) (void)hipFree(b->d_sel);
@@ -549,7 +559,6 @@ int densest_impl(hg_batch* b, const int32_t* graphs, int32_t count, double beta,
             HIP_TRY(dalloc(&b->d_sel, sel.size()));
             b->sel_cap = sel.size();
         }
-        std::vector<BItem> sitems(sel.size());
         for (size_t i = 0; i < sel.size(); ++i) {
             sitems[i] = b->items[sel[i]];
             const BItem& it = sitems[i];
@@ -586,7 +595,14 @@ int densest_impl(hg_batch* b, const int32_t* graphs, int32_t count, double beta,
             HIP_TRY(hipMemcpyAsync(h.data(), b->histpool + it.histb, 4 * h.size(), hipMemcpyDeviceToHost, b->stream));
             HIP_TRY(hipStreamSynchronize(b->stream));
             std::vector<uint64_t> cnt(size_t(km) + 2, 0);
-            for (uint32_t v = 0; v < n; ++v) cnt[size_t(ci[size_t(lmd - 1) * n + v])]++;
+            for (uint32_t v = 0; v < n; ++v) {
+                const int32_t x = ci[size_t(lmd - 1) * n + v];
+                if (x < 0 || x > km)
+                    return fail(HG_ERR_DEVICE, "core index " + std::to_string(x) + " outside [0, " + std::to_string(km) +
+                                                   "] in batch graph " + std::to_string(g) + " (n " + std::to_string(n) +
+                                                   ", L " + std::to_string(L) + ", lmd " + std::to_string(lmd) + ")");
+                cnt[size_t(x)]++;
+            }
             std::vector<uint32_t> acc(L, 0);  // suffix sums over m >= k, uint32 as the reference
             uint64_t len = 0;
             std::vector<double> dens(size_t(km) + 1);
@@ -760,6 +776,7 @@ int hg_batch_cells(hg_batch* b, int32_t graph, int64_t* cell_len, int32_t cap, i
             std::vector<int64_t> h(size_t(cap) + 1, 0);
             for (uint32_t v = 0; v < n; ++v) {
                 const int32_t x = ci[size_t(i) * n + v];
+                if (x < 0) return fail(HG_ERR_DEVICE, "negative core index in a batch result");
                 h[size_t(std::min<int64_t>(x, cap))]++;
             }
             int64_t acc = 0;

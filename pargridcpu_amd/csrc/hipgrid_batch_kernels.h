@@ -92,6 +92,18 @@ __device__ __forceinline__ uint32_t bscan(uint32_t x, uint32_t* total, uint32_t*
     return r;
 }
 
+// Block barrier that also publishes this thread's global-memory writes at
+// device scope.  The item's degree and count words are written with plain
+// stores by one wave and then updated with device-scope atomics, or read
+// with device-scope loads, by others in the same launch; a plain barrier
+// orders them at workgroup scope only, which does not make a store visible
+// to those device-scope operations.  Used at every phase boundary where
+// one wave's stores feed another wave's atomics or loads.
+__device__ __forceinline__ void sync_global() {
+    __threadfence();
+    __syncthreads();
+}
+
 // Degree loads after other threads' atomics: agent-scope loads, not served
 // from a possibly stale vector L1 line.
 __device__ __forceinline__ uint32_t ld_agent(const uint32_t* p) {
@@ -190,7 +202,7 @@ __device__ void peel_level0(const ItemView& v, PeelShared<B>* sh) {
         }
         v.cnt[u] = int32_t(c1);
     }
-    __syncthreads();
+    sync_global();
     // classification, coalesced
     for (uint32_t base = 0; base < n; base += B) {
         const uint32_t u = base + threadIdx.x;
@@ -221,7 +233,7 @@ __device__ void peel_level0(const ItemView& v, PeelShared<B>* sh) {
                 v.queue[qb + __builtin_amdgcn_mbcnt_hi(uint32_t(qm >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(qm), 0u))] = u;
         }
     }
-    __syncthreads();
+    sync_global();
     if (threadIdx.x == 0) sh->nalive = sh->listlen;
     __syncthreads();
 }
@@ -319,7 +331,7 @@ __device__ void peel_scan(const ItemView& v, PeelShared<B>* sh, int32_t k) {
             if (keep) v.list[kb + __builtin_amdgcn_mbcnt_hi(uint32_t(km >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(km), 0u))] = u;
         }
     }
-    __syncthreads();
+    sync_global();  // the cnt stores above feed the cascade's atomics
     if (threadIdx.x == 0) sh->listlen = sh->keep;
     __syncthreads();
 }
@@ -382,6 +394,7 @@ __global__ void __launch_bounds__(B) k_batch_peel(const BGraph* graphs, const BI
             peel_scan<B>(v, &sh, k);
         }
         // every vertex got its index at level <= k; k is the largest index
+        sync_global();
         for (uint32_t u = threadIdx.x; u < v.n; u += B) out[item.outb + v.perm[u]] = v.idx[u];
         if (threadIdx.x == 0) kmax_out[item.id] = k;
         __syncthreads();

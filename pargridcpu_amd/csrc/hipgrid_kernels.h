@@ -68,7 +68,7 @@ constexpr int kWave = 64;
 constexpr int kBlock = 512;  // threads per block of the level kernels (8 wavefronts)
 constexpr int kWavesPerBlock = kBlock / kWave;
 constexpr int kChunk = 64;         // static chunk list granularity (densest pass)
-constexpr uint32_t kHeavy = 4096;  // default: vertices with more entries are deferred
+constexpr uint32_t kHeavy = 2048;  // default: vertices with more entries are deferred
 constexpr uint32_t kDefer = 4096;  // entries per deferred chunk
 constexpr uint32_t kRun = 64;      // consecutive vertex ids per ownership run
 
@@ -912,7 +912,9 @@ __global__ void __launch_bounds__(kBlock) k_idx_hist(const int32_t* idx, uint32_
     const int32_t* row = idx + uint64_t(i) * n;
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
         const int32_t x = row[v];
-        const uint32_t k = x < 0 ? 0u : uint32_t(x);  // every vertex has x >= 0 after a decomposition
+        // every vertex has 0 <= x < kcap after a decomposition; anything else
+        // is left out, so the host's total check reports it
+        const uint32_t k = x < 0 ? kcap : uint32_t(x);
         if (k < lds_k) atomicAdd(&s_h[k], 1u);
         else if (k < kcap) atomicAdd(reinterpret_cast<unsigned long long*>(&hist[uint64_t(i) * kcap + k]), 1ull);
     }

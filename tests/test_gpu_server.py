@@ -116,3 +116,14 @@ def test_concurrent_submitters():
         srv.drain()
         for rid, g in zip(ids, gs):
             check(srv, rid, g, REQ_DECOMPOSE, 1.0)
+
+
+def test_concurrent_densest_batches():
+    """Densest requests in several lanes at once: each lane uploads its
+    batch's item list while the others allocate and free host memory."""
+    gs = request_graphs(24, 0xD5)
+    say("case server concurrent densest batches")
+    with Server(lanes=4, max_batch=3) as srv:
+        ids = [srv.submit(g, REQ_DENSEST, 1.0) for g in gs]
+        for rid, g in zip(ids, gs):
+            check(srv, rid, g, REQ_DENSEST, 1.0)
