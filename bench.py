@@ -41,15 +41,15 @@ def algorithmic_bytes(n: int, L: int, D: int) -> int:
     return L * (12 * D + 8 * L * (n + 1) + 4 * n * L + 8 * n)
 
 
-def measured_traffic(workload: str):
-    """HBM bytes per decomposition from the committed rocprofv3 counter
-    summary of this workload (FETCH_SIZE and WRITE_SIZE runs, calibrated
-    against a 1 GiB copy), or None when no summary exists."""
-    path = os.path.join(ROOT, "profiles", f"hg_decompose_{workload}.md")
+def measured_traffic(name: str, key: str = "traffic_bytes_per_decomposition"):
+    """HBM bytes per launch from the committed rocprofv3 counter summary
+    profiles/<name>.md (FETCH_SIZE and WRITE_SIZE runs, calibrated against
+    a 1 GiB copy), or None when no summary exists."""
+    path = os.path.join(ROOT, "profiles", f"{name}.md")
     if not os.path.exists(path):
         return None, None
     for line in open(path):
-        if line.startswith("traffic_bytes_per_decomposition:"):
+        if line.startswith(key + ":"):
             return int(line.split(":")[1]), os.path.relpath(path, ROOT)
     return None, None
 
@@ -107,11 +107,12 @@ def cpu_baseline(spec, kind: str, budget_s: float = 30.0):
             "seconds": round(secs, 4)}
 
 
-def run_batched(steps: int, count: int = 256) -> dict:
+def run_batched(steps: int, count: int = 256, params: str = "tune") -> dict:
     """Batched form: `count` graphs of the batched workload decomposed in one
-    persistent launch (hg_batch_*, launch shape from its autotuner), against
-    the same graphs decomposed one after another by the single-graph engine
-    (graphs resident, one hg_decompose each).  Results must be identical."""
+    persistent launch (hg_batch_*, launch shape from its autotuner, or fixed
+    by `params` = "block,unroll,blocks_per_cu,order"), against the same graphs
+    decomposed one after another by the single-graph engine (graphs resident,
+    one hg_decompose each).  Results must be identical."""
     import numpy as np
 
     from pargridcpu_amd.graph import MultilayerGraph, batch_workload
@@ -122,8 +123,13 @@ def run_batched(steps: int, count: int = 256) -> dict:
     D = sum(g.directed_entries() for g in gs)
     log(f"[bench] batched: generated {count} graphs (D={D}) in {time.perf_counter() - t0:.1f}s")
     b = Batch(gs)
-    best, tuned_ms = b.tune(reps=1)
-    log(f"[bench] batched: autotuner picked {best} ({tuned_ms:.3f} ms)")
+    if params == "tune":
+        best, tuned_ms = b.tune(reps=1)
+        log(f"[bench] batched: autotuner picked {best} ({tuned_ms:.3f} ms)")
+    else:
+        block, unroll, bpc, order = (int(x) for x in params.split(","))
+        b.set_params(block=block, unroll=unroll, blocks_per_cu=bpc, order=order, use_graph=1)
+        best = b.params().as_dict()
     walls = []
     for _ in range(max(steps, 3)):
         w0 = time.perf_counter()
@@ -142,13 +148,20 @@ def run_batched(steps: int, count: int = 256) -> dict:
     for e in engs:
         e.close()
     b.close()
+    balg = sum(algorithmic_bytes(g.n, g.L, g.directed_entries()) for g in gs)
+    achieved = balg / (dev_ms * 1e-3) / 1e9
+    traffic, traffic_src = (measured_traffic("hg_batch_peel_c4", "traffic_bytes_per_launch") if count == 256
+                            else (None, None))
+    roof = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic, "traffic_source": traffic_src,
+            "kernel": "k_batch_peel (one launch)", "algorithmic_bytes": balg, "avg_ms": round(dev_ms, 4)}
     log(f"[bench] batched: launch {batch_ms:.2f} ms wall ({dev_ms:.2f} ms device), sequential {seq_ms:.1f} ms, "
         f"exact={exact}")
     return {"workload": f"{count} graphs: n log-U[1e3,2e5], L U{{2..16}}, avg deg U[2,20], gamma U[2.1,3.0]",
             "graphs": count, "directed_entries": D, "launch_ms": round(batch_ms, 3), "device_ms": round(dev_ms, 3),
             "graphs_per_s": round(count / (batch_ms * 1e-3), 1), "sequential_ms": round(seq_ms, 3),
             "sequential_graphs_per_s": round(count / (seq_ms * 1e-3), 1), "speedup": round(seq_ms / batch_ms, 2),
-            "exact": bool(exact), "params": best}
+            "exact": bool(exact), "params": best, "roofline": roof}
 
 
 def run_serving(n_requests: int = 160, rate: float = 20.0) -> dict:
@@ -212,29 +225,14 @@ def run_serving(n_requests: int = 160, rate: float = 20.0) -> dict:
     return res
 
 
-def main() -> int:
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default=os.environ.get("HG_WORKLOAD", "c3"))
-    ap.add_argument("--cpu-baseline", default="auto", choices=["auto", "reference", "port", "off"])
-    ap.add_argument("--batched", type=int, default=1, help="also run the batched form (0: skip)")
-    ap.add_argument("--serving", type=int, default=1, help="also run the serving form (0: skip)")
-    ap.add_argument("--calib-copy-mib", type=int, default=0,
-                    help="also run one device-to-device copy of this many MiB (counter calibration)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the lambda-sharded multi-GPU "
-              f"layout is a design, not built", file=sys.stderr)
-        return 2
-
+def run_decompose(args, spec) -> dict:
+    """The headline form: one full decomposition per step of `spec`, graph
+    resident, timed with events on the engine's stream."""
     import torch
 
-    from pargridcpu_amd.graph import WORKLOADS, MultilayerGraph
+    from pargridcpu_amd.graph import MultilayerGraph
     from pargridcpu_amd.hipgrid import Engine
 
-    spec = WORKLOADS[args.workload]
     t0 = time.perf_counter()
     g = MultilayerGraph.generate(spec)
     log(f"[bench] generated {spec.name} in {time.perf_counter() - t0:.1f}s")
@@ -276,7 +274,7 @@ def main() -> int:
     balg = algorithmic_bytes(n, L, D)
     dev_ms = sum(per_call) / len(per_call)
     achieved = balg / (dev_ms * 1e-3) / 1e9
-    traffic, traffic_src = measured_traffic(args.workload)
+    traffic, traffic_src = measured_traffic(f"hg_decompose_{args.workload}")
     roof = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic, "traffic_source": traffic_src,
             "kernel": "hg_decompose (k_init + per-level k_scan/k_cascade)", "algorithmic_bytes": balg,
@@ -290,10 +288,42 @@ def main() -> int:
             cpu = {"value": None, "unit": "GTEPS", "cores": None, "kind": args.cpu_baseline, "sample": f"failed: {e}"}
     eng.close()
     del eng, g
+    return {"value": round(gteps, 4), "ms_per_step": round(ms, 4), "roofline": roof, "cpu_baseline": cpu,
+            "config": {"workload": spec.name, "n": n, "L": L, "directed_entries": D, "gamma": spec.gamma,
+                       "rho": spec.rho, "offsets": "int64", "parallelism": "single-gpu"}}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default=os.environ.get("HG_WORKLOAD", "c3"))
+    ap.add_argument("--cpu-baseline", default="auto", choices=["auto", "reference", "port", "off"])
+    ap.add_argument("--decompose", type=int, default=1, help="run the headline decomposition (0: skip, profiling)")
+    ap.add_argument("--batched", type=int, default=1, help="also run the batched form (0: skip)")
+    ap.add_argument("--batch-params", default="tune",
+                    help='batched launch shape: "tune" or "block,unroll,blocks_per_cu,order"')
+    ap.add_argument("--serving", type=int, default=1, help="also run the serving form (0: skip)")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="also run one device-to-device copy of this many MiB (counter calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the lambda-sharded multi-GPU "
+              f"layout is a design, not built", file=sys.stderr)
+        return 2
+
+    from pargridcpu_amd.graph import WORKLOADS
+
+    spec = WORKLOADS[args.workload]
+    head = {"value": None, "ms_per_step": None, "roofline": None, "cpu_baseline": None,
+            "config": {"workload": spec.name}}
+    if args.decompose:
+        head = run_decompose(args, spec)
     batched = serving = None
     if args.batched:
         try:
-            batched = run_batched(args.steps)
+            batched = run_batched(args.steps, params=args.batch_params)
         except Exception as e:  # reported, never fatal for the headline number
             batched = {"error": str(e)}
     if args.serving:
@@ -302,12 +332,11 @@ def main() -> int:
         except Exception as e:
             serving = {"error": str(e)}
     line = {
-        "metric": METRIC, "value": round(gteps, 4), "unit": "GTEPS", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
+        "metric": METRIC, "value": head["value"], "unit": "GTEPS", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "int32", "data": "synthetic (seeded Chung-Lu multilayer generator)",
-        "config": {"workload": spec.name, "n": n, "L": L, "directed_entries": D, "gamma": spec.gamma, "rho": spec.rho,
-                   "offsets": "int64", "parallelism": "single-gpu"},
-        "roofline": roof, "cpu_baseline": cpu, "batched": batched, "serving": serving,
+        "config": head["config"], "roofline": head["roofline"], "cpu_baseline": head["cpu_baseline"],
+        "batched": batched, "serving": serving,
     }
     print(json.dumps(line), flush=True)
     return 0

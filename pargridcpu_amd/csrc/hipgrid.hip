@@ -29,6 +29,8 @@
 #include <string>
 #include <vector>
 
+#include <hipcub/hipcub.hpp>
+
 #include "hipgrid.h"
 #include "hipgrid_common.h"
 #include "hipgrid_kernels.h"
@@ -89,7 +91,7 @@ struct hg_graph {
     std::vector<double> h_density;   // [L][kcap]
     std::vector<int32_t> h_sel;
     hg_timing timing{};
-    // host sources of small uploads (kept until the next call)
+    // host-side scratch of the per-call uploads
     std::vector<InstState> h_states;
     Control h_ctl_init{};
     std::vector<double> h_pow;
@@ -102,6 +104,7 @@ namespace {
 void release(hg_graph* g) {
     if (!g) return;
     (void)hipSetDevice(g->device);
+    (void)hipStreamSynchronize(g->stream);  // nothing queued may still use the memory
     void* ptrs[] = {g->off, g->adj, g->adjz, g->deg0, g->alive, g->inv, g->z, g->out, g->chunks, g->deg, g->cnt, g->idx, g->lists, g->rings, g->blk, g->pl,
                     g->d_ihist, g->d_insts, g->d_states, g->d_ctl, g->hist, g->d_density, g->d_sel, g->d_cell_len,
                     g->d_pow};
@@ -113,11 +116,42 @@ void release(hg_graph* g) {
     delete g;
 }
 
+// Device memory that lives only through hg_graph_create; freed once the
+// stream has drained (also on an early error return, when copies and
+// kernels that use it may still be queued).
+struct Scratch {
+    hipStream_t stream = nullptr;
+    std::vector<void*> ptrs;
+    ~Scratch() {
+        if (!ptrs.empty()) (void)hipStreamSynchronize(stream);
+        for (void* p : ptrs)
+            if (p) (void)hipFree(p);
+    }
+    template <typename T>
+    hipError_t alloc(T** p, size_t count) {
+        const hipError_t e = dalloc(p, count);
+        if (e == hipSuccess) ptrs.push_back(*p);
+        return e;
+    }
+};
+
+hipError_t sort_by_total(void* temp, size_t& temp_bytes, const uint64_t* tot, uint64_t* tot_s, const uint32_t* ids,
+                         uint32_t* perm, uint32_t n, hipStream_t stream) {
+    // stable: equal totals keep the caller's order
+    return hipcub::DeviceRadixSort::SortPairsDescending(temp, temp_bytes, tot, tot_s, ids, perm, int(n), 0, 40, stream);
+}
+
+// The caller's per-layer CSR goes up as is (through pinned staging); the
+// set-up runs on the device: per-vertex totals, the degree order (a stable
+// radix sort, largest total first), inverse permutation, merged offsets and
+// chunk offsets (prefix sums), then the relabelled vertex-major adjacency.
+// The host reads back a handful of words to size the remaining arrays.
 int create_impl(const hg_layer_csr* layers, int32_t L, int64_t n, int32_t offset_bits, void* stream, hg_graph* g) {
     if (!layers) return fail(HG_ERR_ARG, "layers is NULL");
     if (L < 1 || L > 64) return fail(HG_ERR_RANGE, "L must be in [1, 64]");
     if (n < 1) return fail(HG_ERR_ARG, "n must be >= 1");
     if (uint64_t(n) * uint64_t(L) >= (1ull << 32)) return fail(HG_ERR_RANGE, "n * L must be < 2^32");
+    if (n >= (1ll << 31) - 1) return fail(HG_ERR_RANGE, "n must be < 2^31 - 1");
     if (offset_bits != 32 && offset_bits != 64) return fail(HG_ERR_ARG, "offset_bits must be 32 or 64");
     for (int l = 0; l < L; ++l)
         if (!layers[l].row_ptr || (!layers[l].col && n > 0)) return fail(HG_ERR_ARG, "layer pointer is NULL");
@@ -131,79 +165,113 @@ int create_impl(const hg_layer_csr* layers, int32_t L, int64_t n, int32_t offset
     g->n = uint32_t(n);
     g->L = uint32_t(L);
 
-    // Host pass over the offsets: per-layer int64 offsets, per-vertex totals,
-    // merged offsets, chunk offsets, max layer degree.
-    std::vector<std::vector<int64_t>> rp(static_cast<size_t>(L));
-    uint64_t maxdeg = 0;
+    // Entries per layer from the end offsets (the rest of the offsets are
+    // checked on the device).
+    std::vector<uint64_t> m(static_cast<size_t>(L));
+    uint64_t D = 0;
     for (int l = 0; l < L; ++l) {
-        rp[size_t(l)].resize(size_t(n) + 1);
-        int64_t* dst = rp[size_t(l)].data();
+        int64_t first, last;
         if (offset_bits == 64) {
-            std::memcpy(dst, layers[l].row_ptr, sizeof(int64_t) * (size_t(n) + 1));
+            first = static_cast<const int64_t*>(layers[l].row_ptr)[0];
+            last = static_cast<const int64_t*>(layers[l].row_ptr)[n];
         } else {
-            const int32_t* src = static_cast<const int32_t*>(layers[l].row_ptr);
-            for (int64_t v = 0; v <= n; ++v) dst[v] = src[v];
+            first = static_cast<const int32_t*>(layers[l].row_ptr)[0];
+            last = static_cast<const int32_t*>(layers[l].row_ptr)[n];
         }
-        if (dst[0] != 0) return fail(HG_ERR_ARG, "row_ptr[0] must be 0");
-        for (int64_t v = 0; v < n; ++v) {
-            if (dst[v + 1] < dst[v]) return fail(HG_ERR_ARG, "row_ptr must be non-decreasing");
-            maxdeg = std::max<uint64_t>(maxdeg, uint64_t(dst[v + 1] - dst[v]));
-        }
+        if (first != 0) return fail(HG_ERR_ARG, "row_ptr[0] must be 0");
+        if (last < 0) return fail(HG_ERR_ARG, "row_ptr must be non-decreasing");
+        m[size_t(l)] = uint64_t(last);
+        D += uint64_t(last);
+        if (D >= (1ull << 40)) return fail(HG_ERR_RANGE, "total directed entries must be < 2^40");
     }
-    // Totals and non-empty layer counts in the caller's ids, then the
-    // relabelling: internal id = rank by total degree, descending (counting
-    // sort, stable in the caller's id).
-    std::vector<uint64_t> tot(static_cast<size_t>(n));
-    std::vector<uint8_t> nz_c(static_cast<size_t>(n));
-    uint64_t maxtot = 0;
-    for (int64_t v = 0; v < n; ++v) {
-        uint64_t t = 0;
-        uint32_t z = 0;
-        for (int l = 0; l < L; ++l) {
-            const int64_t d = rp[size_t(l)][size_t(v) + 1] - rp[size_t(l)][size_t(v)];
-            t += uint64_t(d);
-            z += d > 0;
-        }
-        tot[size_t(v)] = t;
-        nz_c[size_t(v)] = uint8_t(z);
-        maxtot = std::max(maxtot, t);
+    g->D = D;
+    const size_t n1 = size_t(n) + 1;
+    const uint32_t grid_n = uint32_t(std::min<uint64_t>((uint64_t(n) + kBlock - 1) / kBlock, uint64_t(g->cus) * 16));
+
+    Scratch tmp;
+    tmp.stream = g->stream;
+    std::vector<int64_t*> d_rp(static_cast<size_t>(L), nullptr);
+    std::vector<uint32_t*> d_col(static_cast<size_t>(L), nullptr);
+    int64_t** d_rp_arr = nullptr;
+    uint32_t** d_col_arr = nullptr;
+    int32_t* d_rp32 = nullptr;
+    uint64_t *tot = nullptr, *tot_s = nullptr, *chunk_cnt = nullptr, *coff = nullptr;
+    uint32_t *ids = nullptr, *perm = nullptr;
+    uint8_t* nzc = nullptr;
+    unsigned long long* prep = nullptr;
+    hgc::Staging up(g->stream);
+    for (int l = 0; l < L; ++l) {
+        HIP_TRY(tmp.alloc(&d_rp[size_t(l)], n1));
+        HIP_TRY(tmp.alloc(&d_col[size_t(l)], m[size_t(l)]));
     }
-    std::vector<uint32_t> perm(static_cast<size_t>(n)), inv(static_cast<size_t>(n));
+    if (offset_bits == 32) HIP_TRY(tmp.alloc(&d_rp32, n1));
+    HIP_TRY(tmp.alloc(&d_rp_arr, size_t(L)));
+    HIP_TRY(tmp.alloc(&d_col_arr, size_t(L)));
+    HIP_TRY(tmp.alloc(&tot, n1));
+    HIP_TRY(tmp.alloc(&tot_s, n1));
+    HIP_TRY(tmp.alloc(&chunk_cnt, n1));
+    HIP_TRY(tmp.alloc(&coff, n1));
+    HIP_TRY(tmp.alloc(&ids, size_t(n)));
+    HIP_TRY(tmp.alloc(&perm, size_t(n)));
+    HIP_TRY(tmp.alloc(&nzc, size_t(n)));
+    HIP_TRY(tmp.alloc(&prep, size_t(kPrepWords)));
+    HIP_TRY(dalloc(&g->off, n1));
+    HIP_TRY(dalloc(&g->inv, size_t(n)));
+    HIP_TRY(dalloc(&g->z, size_t(n)));
+
+    for (int l = 0; l < L; ++l) {
+        if (offset_bits == 64) {
+            HIP_TRY(up.h2d(d_rp[size_t(l)], layers[l].row_ptr, sizeof(int64_t) * n1));
+        } else {  // staged as int32, widened on the device (the copy and kernel are ordered on the stream)
+            HIP_TRY(up.h2d(d_rp32, layers[l].row_ptr, sizeof(int32_t) * n1));
+            hipLaunchKernelGGL(k_widen_offsets, dim3(std::max(grid_n, 1u)), dim3(kBlock), 0, g->stream, d_rp32,
+                               uint64_t(n1), d_rp[size_t(l)]);
+        }
+        if (m[size_t(l)]) HIP_TRY(up.h2d(d_col[size_t(l)], layers[l].col, sizeof(uint32_t) * m[size_t(l)]));
+    }
+    HIP_TRY(up.h2d(d_rp_arr, d_rp.data(), sizeof(int64_t*) * L));
+    HIP_TRY(up.h2d(d_col_arr, d_col.data(), sizeof(uint32_t*) * L));
+    HIP_TRY(hipMemsetAsync(prep, 0, sizeof(unsigned long long) * kPrepWords, g->stream));
+    hipLaunchKernelGGL(k_prep_vertex, dim3(std::max(grid_n, 1u)), dim3(kBlock), 0, g->stream, d_rp_arr, g->n, g->L,
+                       tot, nzc, ids, prep);
+    HIP_TRY(hipGetLastError());
     {
-        std::vector<uint64_t> bucket(size_t(maxtot) + 2, 0);
-        for (int64_t v = 0; v < n; ++v) bucket[size_t(maxtot - tot[size_t(v)]) + 1]++;
-        for (size_t b = 1; b < bucket.size(); ++b) bucket[b] += bucket[b - 1];
-        for (int64_t v = 0; v < n; ++v) {
-            const uint32_t r = uint32_t(bucket[size_t(maxtot - tot[size_t(v)])]++);
-            perm[r] = uint32_t(v);
-            inv[size_t(v)] = r;
-        }
+        size_t tb = 0;
+        HIP_TRY(sort_by_total(nullptr, tb, tot, tot_s, ids, perm, g->n, g->stream));
+        uint8_t* t = nullptr;
+        HIP_TRY(tmp.alloc(&t, std::max<size_t>(tb, 1)));
+        HIP_TRY(sort_by_total(t, tb, tot, tot_s, ids, perm, g->n, g->stream));
     }
-    std::vector<int64_t> off(size_t(n) + 1);
-    std::vector<uint64_t> coff(size_t(n) + 1);
-    std::vector<uint8_t> zr(static_cast<size_t>(n));
-    uint32_t H = 0;
-    off[0] = 0;
-    coff[0] = 0;
-    uint64_t pcap = 0;
-    uint32_t max_nz = 0;
-    for (int64_t r = 0; r < n; ++r) {
-        const uint32_t v = perm[size_t(r)];
-        const uint64_t t = tot[size_t(v)];
-        zr[size_t(r)] = nz_c[v];
-        if (t > kPullHeavy) H = uint32_t(r) + 1;  // totals descend with r
-        max_nz = std::max<uint32_t>(max_nz, nz_c[v]);
-        off[size_t(r) + 1] = off[size_t(r)] + int64_t(t);
-        coff[size_t(r) + 1] = coff[size_t(r)] + (t + kChunk - 1) / kChunk;
-        pcap += (t + kDefer - 1) / kDefer;
+    // the prefix sums run over n + 1 items: the extra last item is 0
+    HIP_TRY(hipMemsetAsync(tot_s + n, 0, sizeof(uint64_t), g->stream));
+    HIP_TRY(hipMemsetAsync(chunk_cnt + n, 0, sizeof(uint64_t), g->stream));
+    hipLaunchKernelGGL((k_prep_rank<uint64_t(kChunk), uint64_t(kDefer), kPullHeavy>), dim3(std::max(grid_n, 1u)),
+                       dim3(kBlock), 0, g->stream, tot_s, perm, nzc, g->n, g->inv, g->z, chunk_cnt, prep);
+    HIP_TRY(hipGetLastError());
+    {
+        uint64_t* off_u = reinterpret_cast<uint64_t*>(g->off);
+        size_t tb1 = 0, tb2 = 0;
+        HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tb1, tot_s, off_u, int(n1), g->stream));
+        HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tb2, chunk_cnt, coff, int(n1), g->stream));
+        uint8_t* t = nullptr;
+        HIP_TRY(tmp.alloc(&t, std::max<size_t>(std::max(tb1, tb2), 1)));
+        size_t tb = std::max(tb1, tb2);
+        HIP_TRY(hipcub::DeviceScan::ExclusiveSum(t, tb, tot_s, off_u, int(n1), g->stream));
+        tb = std::max(tb1, tb2);
+        HIP_TRY(hipcub::DeviceScan::ExclusiveSum(t, tb, chunk_cnt, coff, int(n1), g->stream));
     }
-    std::vector<uint64_t>().swap(tot);
-    g->D = uint64_t(off[size_t(n)]);
-    if (g->D >= (1ull << 40)) return fail(HG_ERR_RANGE, "total directed entries must be < 2^40");
-    g->nchunks = coff[size_t(n)];
-    g->pcap = pcap + 1;
-    g->max_nz = max_nz;
-    g->H = H;
+    unsigned long long h_prep[kPrepWords];
+    uint64_t h_end[2];
+    HIP_TRY(up.d2h(h_prep, prep, sizeof h_prep));
+    HIP_TRY(up.d2h(&h_end[0], g->off + n, sizeof(uint64_t)));
+    HIP_TRY(up.d2h(&h_end[1], coff + n, sizeof(uint64_t)));
+    if (h_prep[kPrepBad] & kBadRowPtr) return fail(HG_ERR_ARG, "row_ptr must be non-decreasing");
+    if (h_end[0] != D) return fail(HG_ERR_DEVICE, "set-up: merged offsets do not sum to the entry count");
+    const uint64_t maxdeg = h_prep[kPrepMaxDeg];
+    g->nchunks = h_end[1];
+    g->pcap = h_prep[kPrepDefer] + 1;
+    g->max_nz = uint32_t(h_prep[kPrepMaxNz]);
+    g->H = uint32_t(h_prep[kPrepHeavy]);
     g->pull = uint32_t(L) <= kPullMaxL && env_int("HG_PULL", 1) == 1 ? 1u : 0u;
     g->kcap = uint32_t(maxdeg + 2);
     // Level-kernel grid: 3 blocks per CU (the cascade's LDS allows 3; a sweep
@@ -217,14 +285,12 @@ int create_impl(const hg_layer_csr* layers, int32_t L, int64_t n, int32_t offset
     const size_t nL = size_t(n) * size_t(L);
     int64_t bytes = 0;
     auto acct = [&](size_t b) { bytes += int64_t(b); };
-    HIP_TRY(dalloc(&g->off, size_t(n) + 1)); acct(8 * (size_t(n) + 1));
+    acct(8 * n1 + 4 * size_t(n) + size_t(n));  // off, inv, z
     HIP_TRY(dalloc(&g->adj, g->D)); acct(4 * g->D);
     HIP_TRY(dalloc(&g->adjz, g->D)); acct(g->D);
     HIP_TRY(dalloc(&g->deg0, nL)); acct(4 * nL);
     const size_t aw = (size_t(n) + 31) / 32;
     HIP_TRY(dalloc(&g->alive, aw * L)); acct(4 * aw * L);
-    HIP_TRY(dalloc(&g->inv, size_t(n))); acct(4 * size_t(n));
-    HIP_TRY(dalloc(&g->z, size_t(n))); acct(size_t(n));
     HIP_TRY(dalloc(&g->out, nL)); acct(4 * nL);
     HIP_TRY(dalloc(&g->chunks, g->nchunks)); acct(8 * g->nchunks);
     HIP_TRY(dalloc(&g->deg, nL * L)); acct(4 * nL * L);
@@ -247,76 +313,35 @@ int create_impl(const hg_layer_csr* layers, int32_t L, int64_t n, int32_t offset
     for (auto& e : g->ev) HIP_TRY(hipEventCreate(&e));
     g->device_bytes = bytes;
 
-    // Upload per-layer CSR into temporaries, build deg0 / adj / chunks.
-    std::vector<int64_t*> d_rp(static_cast<size_t>(L), nullptr);
-    std::vector<uint32_t*> d_col(static_cast<size_t>(L), nullptr);
-    int64_t** d_rp_arr = nullptr;
-    uint32_t** d_col_arr = nullptr;
-    uint64_t* d_coff = nullptr;
-    uint32_t* d_perm = nullptr;
-    uint8_t* d_zc = nullptr;
-    auto cleanup = [&]() {
-        if (d_perm) (void)hipFree(d_perm);
-        if (d_zc) (void)hipFree(d_zc);
-        for (auto p : d_rp) if (p) (void)hipFree(p);
-        for (auto p : d_col) if (p) (void)hipFree(p);
-        if (d_rp_arr) (void)hipFree(d_rp_arr);
-        if (d_col_arr) (void)hipFree(d_col_arr);
-        if (d_coff) (void)hipFree(d_coff);
-    };
-    hipError_t e = hipSuccess;
-    for (int l = 0; l < L && e == hipSuccess; ++l) {
-        const size_t m = size_t(rp[size_t(l)][size_t(n)]);
-        e = dalloc(&d_rp[size_t(l)], size_t(n) + 1);
-        if (e == hipSuccess) e = dalloc(&d_col[size_t(l)], m);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(d_rp[size_t(l)], rp[size_t(l)].data(), sizeof(int64_t) * (size_t(n) + 1),
-                               hipMemcpyHostToDevice, g->stream);
-        if (e == hipSuccess && m)
-            e = hipMemcpyAsync(d_col[size_t(l)], layers[l].col, sizeof(uint32_t) * m, hipMemcpyHostToDevice,
-                               g->stream);
+    // deg0 / adj / chunks in internal ids
+    const uint32_t grid = uint32_t(std::min<uint64_t>((nL + kBlock - 1) / kBlock, uint64_t(g->cus) * 16));
+    hipLaunchKernelGGL(k_build_deg0, dim3(std::max(grid, 1u)), dim3(kBlock), 0, g->stream, d_rp_arr, perm, g->n, g->L,
+                       g->deg0);
+    const uint32_t grid2 = uint32_t(std::min<uint64_t>((uint64_t(n) * kWave + kBlock - 1) / kBlock,
+                                                       uint64_t(g->cus) * 16));
+    hipLaunchKernelGGL(k_build_adj, dim3(std::max(grid2, 1u)), dim3(kBlock), 0, g->stream, d_rp_arr, d_col_arr, g->deg0,
+                       g->off, perm, g->inv, nzc, g->n, g->L, g->adj, g->adjz, prep);
+    hipLaunchKernelGGL(k_build_chunks, dim3(std::max(grid_n, 1u)), dim3(kBlock), 0, g->stream, g->off, coff, g->n,
+                       g->chunks);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(up.d2h(h_prep, prep, sizeof h_prep));  // waits for the build kernels
+    if (h_prep[kPrepBad] & kBadCol) return fail(HG_ERR_ARG, "a neighbour id is >= n");
+    if (env_int("HG_VERIFY_UPLOAD", 0) == 1) {  // diagnostic: read every layer back
+        for (int l = 0; l < L; ++l) {
+            const size_t ml = size_t(m[size_t(l)]);
+            std::vector<int64_t> brp(n1);
+            std::vector<uint32_t> bcol(ml);
+            HIP_TRY(up.d2h(brp.data(), d_rp[size_t(l)], sizeof(int64_t) * n1));
+            if (ml) HIP_TRY(up.d2h(bcol.data(), d_col[size_t(l)], sizeof(uint32_t) * ml));
+            bool same = ml == 0 || std::memcmp(bcol.data(), layers[l].col, sizeof(uint32_t) * ml) == 0;
+            for (size_t v = 0; v < n1 && same; ++v)
+                same = brp[v] == (offset_bits == 64 ? static_cast<const int64_t*>(layers[l].row_ptr)[v]
+                                                    : int64_t(static_cast<const int32_t*>(layers[l].row_ptr)[v]));
+            if (!same) return fail(HG_ERR_HIP, "upload verify: layer " + std::to_string(l) + " differs on the device");
+        }
     }
-    if (e == hipSuccess) e = dalloc(&d_rp_arr, size_t(L));
-    if (e == hipSuccess) e = dalloc(&d_col_arr, size_t(L));
-    if (e == hipSuccess) e = dalloc(&d_coff, size_t(n) + 1);
-    if (e == hipSuccess) e = dalloc(&d_perm, size_t(n));
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(d_perm, perm.data(), sizeof(uint32_t) * size_t(n), hipMemcpyHostToDevice, g->stream);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(g->inv, inv.data(), sizeof(uint32_t) * size_t(n), hipMemcpyHostToDevice, g->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(g->z, zr.data(), size_t(n), hipMemcpyHostToDevice, g->stream);
-    if (e == hipSuccess) e = dalloc(&d_zc, size_t(n));
-    if (e == hipSuccess) e = hipMemcpyAsync(d_zc, nz_c.data(), size_t(n), hipMemcpyHostToDevice, g->stream);
-
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(d_rp_arr, d_rp.data(), sizeof(int64_t*) * L, hipMemcpyHostToDevice, g->stream);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(d_col_arr, d_col.data(), sizeof(uint32_t*) * L, hipMemcpyHostToDevice, g->stream);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(g->off, off.data(), sizeof(int64_t) * (size_t(n) + 1), hipMemcpyHostToDevice, g->stream);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(d_coff, coff.data(), sizeof(uint64_t) * (size_t(n) + 1), hipMemcpyHostToDevice, g->stream);
-    if (e == hipSuccess) {
-        const uint32_t grid = uint32_t(std::min<uint64_t>((nL + kBlock - 1) / kBlock, uint64_t(g->cus) * 16));
-        hipLaunchKernelGGL(k_build_deg0, dim3(grid), dim3(kBlock), 0, g->stream, d_rp_arr, d_perm, g->n, g->L,
-                           g->deg0);
-        const uint32_t grid2 = uint32_t(std::min<uint64_t>((uint64_t(n) * kWave + kBlock - 1) / kBlock,
-                                                           uint64_t(g->cus) * 16));
-        hipLaunchKernelGGL(k_build_adj, dim3(grid2), dim3(kBlock), 0, g->stream, d_rp_arr, d_col_arr, g->deg0,
-                           g->off, d_perm, g->inv, d_zc, g->n, g->L, g->adj, g->adjz);
-        const uint32_t grid3 = uint32_t(std::min<uint64_t>((uint64_t(n) + kBlock - 1) / kBlock, uint64_t(g->cus) * 16));
-        hipLaunchKernelGGL(k_build_chunks, dim3(grid3), dim3(kBlock), 0, g->stream, g->off, d_coff, g->n, g->chunks);
-        e = hipGetLastError();
-    }
-    // also on failure: queued copies read the host vectors and temporaries freed below
-    {
-        const hipError_t es = hipStreamSynchronize(g->stream);
-        if (e == hipSuccess) e = es;
-    }
-    cleanup();
-    if (e != hipSuccess)
-        return fail(e == hipErrorOutOfMemory ? HG_ERR_NOMEM : HG_ERR_HIP,
-                    std::string("graph upload: ") + hipGetErrorString(e));
+    // the temporaries are freed on return: nothing queued reads them any more
+    HIP_TRY(hipStreamSynchronize(g->stream));
 
     g->insts.resize(size_t(L));
     for (uint32_t i = 0; i < g->L; ++i) {
@@ -384,8 +409,6 @@ int decompose_impl(hg_graph* g) {
     const uint32_t L = g->L;
     g->decomposed = false;
     g->densest_done = false;
-    // host sources of the uploads below live in the handle: a queued copy
-    // from pageable memory may read them after this call has returned early
     std::vector<InstState>& st = g->h_states;
     st.assign(L, InstState{});
     std::memset(st.data(), 0, sizeof(InstState) * L);
@@ -404,9 +427,10 @@ int decompose_impl(hg_graph* g) {
     Control& ctl = g->h_ctl_init;
     std::memset(&ctl, 0, sizeof ctl);
     ctl.active = int32_t(L);
-    HIP_TRY(hipMemcpyAsync(g->d_insts, g->insts.data(), sizeof(Inst) * L, hipMemcpyHostToDevice, g->stream));
-    HIP_TRY(hipMemcpyAsync(g->d_states, st.data(), sizeof(InstState) * L, hipMemcpyHostToDevice, g->stream));
-    HIP_TRY(hipMemcpyAsync(g->d_ctl, &ctl, sizeof(Control), hipMemcpyHostToDevice, g->stream));
+    hgc::Staging io(g->stream);
+    HIP_TRY(io.h2d(g->d_insts, g->insts.data(), sizeof(Inst) * L));
+    HIP_TRY(io.h2d(g->d_states, st.data(), sizeof(InstState) * L));
+    HIP_TRY(io.h2d(g->d_ctl, &ctl, sizeof(Control)));
 
     HIP_TRY(hipEventRecord(g->ev[0], g->stream));
     uint32_t s = 0;
@@ -450,9 +474,7 @@ int decompose_impl(hg_graph* g) {
     g->timing.launches = int64_t(s) * 2 + 3;
 
     g->h_count.assign(size_t(g->kcap) * L, 0);
-    HIP_TRY(hipMemcpyAsync(g->h_count.data(), g->d_ihist, sizeof(int64_t) * g->h_count.size(), hipMemcpyDeviceToHost,
-                           g->stream));
-    HIP_TRY(hipStreamSynchronize(g->stream));
+    HIP_TRY(io.d2h(g->h_count.data(), g->d_ihist, sizeof(int64_t) * g->h_count.size()));
     g->h_kmax.assign(L, 0);
     int64_t total = 0;
     for (uint32_t i = 0; i < L; ++i) {
@@ -484,10 +506,11 @@ std::vector<int64_t> cell_lengths(const hg_graph* g, uint32_t i) {
 int densest_impl(hg_graph* g, double beta, hg_wds* out) {
     HIP_TRY(hipSetDevice(g->device));
     const uint32_t L = g->L;
-    std::vector<double>& pw = g->h_pow;  // lives in the handle (source of a queued copy)
+    std::vector<double>& pw = g->h_pow;
     pw.assign(L, 0.0);
     for (uint32_t l = 0; l < L; ++l) pw[l] = std::pow(double(l + 1), beta);
-    HIP_TRY(hipMemcpyAsync(g->d_pow, pw.data(), sizeof(double) * L, hipMemcpyHostToDevice, g->stream));
+    hgc::Staging io(g->stream);
+    HIP_TRY(io.h2d(g->d_pow, pw.data(), sizeof(double) * L));
     int32_t kmax_all = 0;
     for (uint32_t i = 0; i < L; ++i) kmax_all = std::max(kmax_all, g->h_kmax[i]);
     const size_t kh = size_t(kmax_all) + 1;
@@ -507,19 +530,15 @@ int densest_impl(hg_graph* g, double beta, hg_wds* out) {
         HIP_TRY(hipMemsetAsync(g->hist, 0, sizeof(uint64_t) * size_t(L) * kc, g->stream));
         hipLaunchKernelGGL(k_edge_hist, dim3(std::max(grid, 1u)), dim3(kHistBlock), 0, g->stream, g->insts[i], g->chunks,
                            g->nchunks, g->hist, kc);
-        std::vector<int64_t>& len = g->h_len;  // lives in the handle (source of a queued copy)
+        std::vector<int64_t>& len = g->h_len;
         len = cell_lengths(g, i);
-        HIP_TRY(hipMemcpyAsync(g->d_cell_len, len.data(), sizeof(int64_t) * len.size(), hipMemcpyHostToDevice,
-                               g->stream));
+        HIP_TRY(io.h2d(g->d_cell_len, len.data(), sizeof(int64_t) * len.size()));
         const uint32_t cells = uint32_t(km);
         hipLaunchKernelGGL(k_cell_score, dim3((cells + 63) / 64), dim3(64), 0, g->stream, g->hist, kc, L, uint32_t(km),
                            g->d_cell_len, g->d_pow, g->d_density, g->d_sel);
         HIP_TRY(hipGetLastError());
-        HIP_TRY(hipMemcpyAsync(g->h_density.data() + size_t(i) * g->kcap, g->d_density, sizeof(double) * (size_t(km) + 1),
-                               hipMemcpyDeviceToHost, g->stream));
-        HIP_TRY(hipMemcpyAsync(g->h_sel.data() + size_t(i) * g->kcap, g->d_sel, sizeof(int32_t) * (size_t(km) + 1),
-                               hipMemcpyDeviceToHost, g->stream));
-        HIP_TRY(hipStreamSynchronize(g->stream));  // host buffers are reused per instance
+        HIP_TRY(io.d2h(g->h_density.data() + size_t(i) * g->kcap, g->d_density, sizeof(double) * (size_t(km) + 1)));
+        HIP_TRY(io.d2h(g->h_sel.data() + size_t(i) * g->kcap, g->d_sel, sizeof(int32_t) * (size_t(km) + 1)));
         g->h_density[size_t(i) * g->kcap] = 0.0;
         g->h_sel[size_t(i) * g->kcap] = 0;
     }
@@ -580,6 +599,7 @@ int densest_coreindex_impl(hg_graph* g, double beta, hg_wds* answer, hg_wds* per
     std::vector<hg_wds> rec(size_t(L) + 1);
     for (auto& r : rec) std::memset(&r, 0, sizeof r);
     std::vector<uint64_t> h;
+    hgc::Staging io(g->stream);
     for (uint32_t i = 0; i < L; ++i) {
         const int32_t km = g->h_kmax[i];
         hg_wds& r = rec[size_t(i) + 1];
@@ -590,8 +610,7 @@ int densest_coreindex_impl(hg_graph* g, double beta, hg_wds* answer, hg_wds* per
                            g->nchunks, g->hist, kc);
         HIP_TRY(hipGetLastError());
         h.assign(size_t(L) * kc, 0);
-        HIP_TRY(hipMemcpyAsync(h.data(), g->hist, sizeof(uint64_t) * h.size(), hipMemcpyDeviceToHost, g->stream));
-        HIP_TRY(hipStreamSynchronize(g->stream));
+        HIP_TRY(io.d2h(h.data(), g->hist, sizeof(uint64_t) * h.size()));
         const std::vector<int64_t> len = cell_lengths(g, i);
         std::vector<uint64_t> acc(L, 0);
         std::vector<float> dens(size_t(km) + 1, 0.0f);
@@ -676,8 +695,8 @@ int hg_core_index(hg_graph* g, int32_t* core_index) {
     if (!g || !core_index) return fail(HG_ERR_ARG, "NULL argument");
     if (!g->decomposed) return fail(HG_ERR_STATE, "hg_decompose has not completed");
     HIP_TRY(hipSetDevice(g->device));
-    HIP_TRY(hipMemcpyAsync(core_index, g->out, sizeof(int32_t) * size_t(g->n) * g->L, hipMemcpyDeviceToHost, g->stream));
-    HIP_TRY(hipStreamSynchronize(g->stream));
+    hgc::Staging io(g->stream);
+    HIP_TRY(io.d2h(core_index, g->out, sizeof(int32_t) * size_t(g->n) * g->L));
     return HG_OK;
 }
 
@@ -775,9 +794,8 @@ int hg_densest_vertices(hg_graph* g, uint32_t* out, int64_t cap, int64_t* count)
     try {
         std::vector<int32_t> row(g->n);
         HIP_TRY(hipSetDevice(g->device));
-        HIP_TRY(hipMemcpyAsync(row.data(), g->out + size_t(g->wds.res_lmd - 1) * g->n, sizeof(int32_t) * g->n,
-                               hipMemcpyDeviceToHost, g->stream));
-        HIP_TRY(hipStreamSynchronize(g->stream));
+        hgc::Staging io(g->stream);
+        HIP_TRY(io.d2h(row.data(), g->out + size_t(g->wds.res_lmd - 1) * g->n, sizeof(int32_t) * g->n));
         int64_t c = 0;
         for (uint32_t v = 0; v < g->n; ++v) {
             if (row[v] >= int32_t(g->wds.res_k)) {

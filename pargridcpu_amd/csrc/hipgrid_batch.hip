@@ -354,8 +354,9 @@ int create_impl(const hg_batch_graph* gs, int32_t count, void* stream, hg_batch*
     int64_t* d_off64 = nullptr;
     uint32_t* d_inv = nullptr;
     uint8_t* d_zc = nullptr;
+    unsigned long long* d_prep = nullptr;
     auto cleanup = [&]() {
-        void* t[] = {d_rp, d_col, d_rp_ptr, d_col_ptr, d_off64, d_inv, d_zc};
+        void* t[] = {d_rp, d_col, d_rp_ptr, d_col_ptr, d_off64, d_inv, d_zc, d_prep};
         for (void* p : t)
             if (p) (void)hipFree(p);
     };
@@ -366,8 +367,11 @@ int create_impl(const hg_batch_graph* gs, int32_t count, void* stream, hg_batch*
     if (e == hipSuccess) e = dalloc(&d_off64, O);
     if (e == hipSuccess) e = dalloc(&d_inv, V);
     if (e == hipSuccess) e = dalloc(&d_zc, V);
+    if (e == hipSuccess) e = dalloc(&d_prep, size_t(hg::kPrepWords));
+    if (e == hipSuccess) e = hipMemsetAsync(d_prep, 0, sizeof(unsigned long long) * hg::kPrepWords, b->stream);
     std::vector<int64_t*> rp_ptr(LAY);
     std::vector<uint32_t*> col_ptr(LAY);
+    hgc::Staging up(b->stream);
     if (e == hipSuccess) {
         uint64_t cpos = 0;
         for (size_t g = 0; g < G; ++g) {
@@ -376,24 +380,22 @@ int create_impl(const hg_batch_graph* gs, int32_t count, void* stream, hg_batch*
                 rp_ptr[layb[g] + l] = d_rp + rpb[g] + uint64_t(l) * (n + 1);
                 col_ptr[layb[g] + l] = d_col + cpos;
                 const size_t m = size_t(prep[g].rp[l][n]);
-                if (m && e == hipSuccess)
-                    e = hipMemcpyAsync(d_col + cpos, gs[g].layers[l].col, 4 * m, hipMemcpyHostToDevice, b->stream);
+                if (m && e == hipSuccess) e = up.h2d(d_col + cpos, gs[g].layers[l].col, 4 * m);
                 cpos += m;
             }
         }
     }
-    if (e == hipSuccess) e = hipMemcpyAsync(d_rp, h_rp.data(), 8 * RP, hipMemcpyHostToDevice, b->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(d_rp_ptr, rp_ptr.data(), sizeof(int64_t*) * LAY, hipMemcpyHostToDevice, b->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(d_col_ptr, col_ptr.data(), sizeof(uint32_t*) * LAY, hipMemcpyHostToDevice, b->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(d_off64, h_off64.data(), 8 * O, hipMemcpyHostToDevice, b->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(d_inv, h_inv.data(), 4 * V, hipMemcpyHostToDevice, b->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(d_zc, h_zc.data(), V, hipMemcpyHostToDevice, b->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(b->off, h_off.data(), 4 * O, hipMemcpyHostToDevice, b->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(b->z, h_z.data(), V, hipMemcpyHostToDevice, b->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(b->perm, h_perm.data(), 4 * V, hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess) e = up.h2d(d_rp, h_rp.data(), 8 * RP);
+    if (e == hipSuccess) e = up.h2d(d_rp_ptr, rp_ptr.data(), sizeof(int64_t*) * LAY);
+    if (e == hipSuccess) e = up.h2d(d_col_ptr, col_ptr.data(), sizeof(uint32_t*) * LAY);
+    if (e == hipSuccess) e = up.h2d(d_off64, h_off64.data(), 8 * O);
+    if (e == hipSuccess) e = up.h2d(d_inv, h_inv.data(), 4 * V);
+    if (e == hipSuccess) e = up.h2d(d_zc, h_zc.data(), V);
+    if (e == hipSuccess) e = up.h2d(b->off, h_off.data(), 4 * O);
+    if (e == hipSuccess) e = up.h2d(b->z, h_z.data(), V);
+    if (e == hipSuccess) e = up.h2d(b->perm, h_perm.data(), 4 * V);
     if (e == hipSuccess) e = hipMemsetAsync(b->out, 0, 4 * std::max<uint64_t>(OUT, 1), b->stream);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(b->d_graphs, b->graphs.data(), sizeof(BGraph) * G, hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess) e = up.h2d(b->d_graphs, b->graphs.data(), sizeof(BGraph) * G);
     for (size_t g = 0; g < G && e == hipSuccess; ++g) {
         const uint64_t nL = uint64_t(b->n[g]) * b->L[g];
         const uint32_t grid = uint32_t(std::min<uint64_t>((nL + hg::kBlock - 1) / hg::kBlock, uint64_t(b->cus) * 16));
@@ -403,13 +405,38 @@ int create_impl(const hg_batch_graph* gs, int32_t count, void* stream, hg_batch*
                                                            uint64_t(b->cus) * 16));
         hipLaunchKernelGGL(hg::k_build_adj, dim3(std::max(grid2, 1u)), dim3(hg::kBlock), 0, b->stream,
                            d_rp_ptr + layb[g], d_col_ptr + layb[g], b->deg0 + db[g], d_off64 + ob[g], b->perm + b->vb[g],
-                           d_inv + b->vb[g], d_zc + b->vb[g], b->n[g], b->L[g], b->adj + eb[g], b->adjz + eb[g]);
+                           d_inv + b->vb[g], d_zc + b->vb[g], b->n[g], b->L[g], b->adj + eb[g], b->adjz + eb[g],
+                           d_prep);
         e = hipGetLastError();
     }
-    // also on failure: queued copies read the host vectors and temporaries freed below
+    // also on failure: queued kernels read the temporaries freed below
     {
         const hipError_t es = hipStreamSynchronize(b->stream);
         if (e == hipSuccess) e = es;
+    }
+    unsigned long long h_prep[hg::kPrepWords] = {};
+    if (e == hipSuccess) e = up.d2h(h_prep, d_prep, sizeof h_prep);
+    if (e == hipSuccess && (h_prep[hg::kPrepBad] & hg::kBadCol)) {
+        cleanup();
+        return fail(HG_ERR_ARG, "a neighbour id is >= n in a batch graph");
+    }
+    if (e == hipSuccess && env_int("HG_VERIFY_UPLOAD", 0) == 1) {  // diagnostic: read every layer back
+        std::vector<int64_t> brp(RP);
+        e = up.d2h(brp.data(), d_rp, 8 * RP);
+        bool same = e == hipSuccess && std::memcmp(brp.data(), h_rp.data(), 8 * RP) == 0;
+        uint64_t cpos = 0;
+        for (size_t g = 0; g < G && same && e == hipSuccess; ++g)
+            for (uint32_t l = 0; l < b->L[g] && same && e == hipSuccess; ++l) {
+                const size_t m = size_t(prep[g].rp[l][b->n[g]]);
+                std::vector<uint32_t> bcol(m);
+                if (m) e = up.d2h(bcol.data(), d_col + cpos, 4 * m);
+                same = e == hipSuccess && (m == 0 || std::memcmp(bcol.data(), gs[g].layers[l].col, 4 * m) == 0);
+                cpos += m;
+            }
+        if (e == hipSuccess && !same) {
+            cleanup();
+            return fail(HG_ERR_HIP, "upload verify: a graph's layers differ on the device");
+        }
     }
     cleanup();
     if (e != hipSuccess)
@@ -426,8 +453,8 @@ int upload_items(hg_batch* b) {
         std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t c) { return b->cost[a] > b->cost[c]; });
         for (size_t i = 0; i < ord.size(); ++i) q[i] = b->items[ord[i]];
     }
-    if (!q.empty())
-        HIP_TRY(hipMemcpyAsync(b->d_items, q.data(), sizeof(BItem) * q.size(), hipMemcpyHostToDevice, b->stream));
+    hgc::Staging up(b->stream);
+    if (!q.empty()) HIP_TRY(up.h2d(b->d_items, q.data(), sizeof(BItem) * q.size()));
     HIP_TRY(hipStreamSynchronize(b->stream));
     b->items_uploaded = true;
     return HG_OK;
@@ -489,9 +516,8 @@ int decompose_impl(hg_batch* b) {
     HIP_TRY(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
     b->decompose_ms = ms;
     b->h_kmax.assign(b->items.size(), 0);
-    HIP_TRY(hipMemcpyAsync(b->h_kmax.data(), b->d_kmax, sizeof(int32_t) * b->items.size(), hipMemcpyDeviceToHost,
-                           b->stream));
-    HIP_TRY(hipStreamSynchronize(b->stream));
+    hgc::Staging io(b->stream);
+    HIP_TRY(io.d2h(b->h_kmax.data(), b->d_kmax, sizeof(int32_t) * b->items.size()));
     b->decomposed = true;
     return HG_OK;
 }
@@ -547,9 +573,7 @@ int densest_impl(hg_batch* b, const int32_t* graphs, int32_t count, double beta,
     std::sort(sel.begin(), sel.end());
     sel.erase(std::unique(sel.begin(), sel.end()), sel.end());
     HIP_TRY(hipEventRecord(b->ev[0], b->stream));
-    // Host source of the item upload below: an asynchronous copy from
-    // pageable memory may read it when the stream reaches the copy, so it
-    // lives until the stream is synchronised.
+    hgc::Staging io(b->stream);
     std::vector<BItem> sitems(sel.size());
     if (!sel.empty()) {
         if (sel.size() > b->sel_cap) {
@@ -564,7 +588,7 @@ int densest_impl(hg_batch* b, const int32_t* graphs, int32_t count, double beta,
             const BItem& it = sitems[i];
             HIP_TRY(hipMemsetAsync(b->histpool + it.histb, 0, 4 * uint64_t(b->L[it.g]) * it.kcap, b->stream));
         }
-        HIP_TRY(hipMemcpyAsync(b->d_sel, sitems.data(), sizeof(BItem) * sel.size(), hipMemcpyHostToDevice, b->stream));
+        HIP_TRY(io.h2d(b->d_sel, sitems.data(), sizeof(BItem) * sel.size()));
         const uint32_t grid = uint32_t(std::min<size_t>(sel.size(), size_t(b->cus) * 4));
         hipLaunchKernelGGL((k_batch_edge_hist<256>), dim3(grid), dim3(256), 0, b->stream, b->d_graphs, b->d_sel,
                            uint32_t(sel.size()), b->off, b->adj, b->idxpool, b->histpool);
@@ -583,8 +607,7 @@ int densest_impl(hg_batch* b, const int32_t* graphs, int32_t count, double beta,
         std::vector<double> pw(L);
         for (uint32_t l = 0; l < L; ++l) pw[l] = std::pow(double(l + 1), beta);
         std::vector<int32_t> ci(size_t(n) * L);
-        HIP_TRY(hipMemcpyAsync(ci.data(), b->out + b->outb[size_t(g)], 4 * ci.size(), hipMemcpyDeviceToHost, b->stream));
-        HIP_TRY(hipStreamSynchronize(b->stream));
+        HIP_TRY(io.d2h(ci.data(), b->out + b->outb[size_t(g)], 4 * ci.size()));
         hg_wds r{};
         float maxd = 0.0f;
         for (uint32_t lmd = 1; lmd <= L; ++lmd) {
@@ -592,8 +615,7 @@ int densest_impl(hg_batch* b, const int32_t* graphs, int32_t count, double beta,
             if (km < 1) continue;
             const BItem& it = b->items[size_t(b->item_of[size_t(g)][lmd - 1])];
             std::vector<uint32_t> h(size_t(L) * it.kcap);
-            HIP_TRY(hipMemcpyAsync(h.data(), b->histpool + it.histb, 4 * h.size(), hipMemcpyDeviceToHost, b->stream));
-            HIP_TRY(hipStreamSynchronize(b->stream));
+            HIP_TRY(io.d2h(h.data(), b->histpool + it.histb, 4 * h.size()));
             std::vector<uint64_t> cnt(size_t(km) + 2, 0);
             for (uint32_t v = 0; v < n; ++v) {
                 const int32_t x = ci[size_t(lmd - 1) * n + v];
@@ -751,10 +773,9 @@ int hg_batch_core_index(hg_batch* b, int32_t graph, int32_t* core_index) {
     if (rc) return rc;
     if (!core_index) return fail(HG_ERR_ARG, "core_index is NULL");
     HIP_TRY(hipSetDevice(b->device));
-    HIP_TRY(hipMemcpyAsync(core_index, b->out + b->outb[size_t(graph)],
-                           sizeof(int32_t) * uint64_t(b->n[size_t(graph)]) * b->L[size_t(graph)], hipMemcpyDeviceToHost,
-                           b->stream));
-    HIP_TRY(hipStreamSynchronize(b->stream));
+    hgc::Staging io(b->stream);
+    HIP_TRY(io.d2h(core_index, b->out + b->outb[size_t(graph)],
+                   sizeof(int32_t) * uint64_t(b->n[size_t(graph)]) * b->L[size_t(graph)]));
     return HG_OK;
 }
 
@@ -769,9 +790,8 @@ int hg_batch_cells(hg_batch* b, int32_t graph, int64_t* cell_len, int32_t cap, i
     try {
         std::vector<int32_t> ci(size_t(n) * L);
         HIP_TRY(hipSetDevice(b->device));
-        HIP_TRY(hipMemcpyAsync(ci.data(), b->out + b->outb[size_t(graph)], 4 * ci.size(), hipMemcpyDeviceToHost,
-                               b->stream));
-        HIP_TRY(hipStreamSynchronize(b->stream));
+        hgc::Staging io(b->stream);
+        HIP_TRY(io.d2h(ci.data(), b->out + b->outb[size_t(graph)], 4 * ci.size()));
         for (uint32_t i = 0; i < L; ++i) {
             std::vector<int64_t> h(size_t(cap) + 1, 0);
             for (uint32_t v = 0; v < n; ++v) {

@@ -44,6 +44,8 @@ namespace hgb {
 using hg::div_L;
 using hg::kWave;
 using hg::lane_id;
+using hg::ld_agent;
+using hg::sync_stores;
 using hg::wave_excl_scan;
 
 constexpr uint32_t kLdsBits = 1u << 18;        // vertices whose liveness bitmap fits in LDS (32 KB)
@@ -104,11 +106,6 @@ __device__ __forceinline__ void sync_global() {
     __syncthreads();
 }
 
-// Degree loads after other threads' atomics: agent-scope loads, not served
-// from a possibly stale vector L1 line.
-__device__ __forceinline__ uint32_t ld_agent(const uint32_t* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
 
 template <int B>
 struct PeelShared {
@@ -245,13 +242,13 @@ template <int B, int U>
 __device__ void peel_cascade(const ItemView& v, PeelShared<B>* sh, int32_t k) {
     const uint32_t thr = uint32_t(k) + 1u, lmd = v.lmd;
     for (;;) {
-        __syncthreads();
+        sync_stores();  // FIFO entries, liveness bits and degrees of the last round are settled
         const uint32_t qh = sh->qhead, qt = sh->qtail < v.n ? sh->qtail : v.n;
         if (qh >= qt) break;
         const uint32_t nb = qt - qh < uint32_t(B) ? qt - qh : uint32_t(B);
         uint32_t start = 0, len = 0;
         if (threadIdx.x < nb) {
-            const uint32_t u = v.queue[qh + threadIdx.x];
+            const uint32_t u = ld_agent(&v.queue[qh + threadIdx.x]);
             start = v.off[u];
             len = v.off[u + 1] - start;
         }
@@ -312,7 +309,7 @@ __device__ void peel_scan(const ItemView& v, PeelShared<B>* sh, int32_t k) {
         uint32_t u = 0;
         bool live = false;
         if (t < len) {
-            u = v.list[t];
+            u = ld_agent(&v.list[t]);
             live = bit_get(v, u);
         }
         uint32_t c = 0;
@@ -395,7 +392,8 @@ __global__ void __launch_bounds__(B) k_batch_peel(const BGraph* graphs, const BI
         }
         // every vertex got its index at level <= k; k is the largest index
         sync_global();
-        for (uint32_t u = threadIdx.x; u < v.n; u += B) out[item.outb + v.perm[u]] = v.idx[u];
+        for (uint32_t u = threadIdx.x; u < v.n; u += B)
+            out[item.outb + v.perm[u]] = int32_t(ld_agent(reinterpret_cast<const uint32_t*>(&v.idx[u])));
         if (threadIdx.x == 0) kmax_out[item.id] = k;
         __syncthreads();
     }

@@ -131,6 +131,23 @@ enum Err : int32_t { kErrNone = 0, kErrDefer = 2 };
 // ----------------------------------------------------------------- helpers
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
+// Waves of one block hand each other work through global memory (a block's
+// removal ring, a batch item's FIFO and alive list).  A plain block barrier
+// waits for LDS operations only; a global store issued before it can still
+// be in flight when another wave reads the word after it.  This barrier
+// first waits for every memory operation the thread has issued (stores and
+// atomics included), so they are complete at the L2 when any wave passes.
+__device__ __forceinline__ void sync_stores() {
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+}
+
+// Agent-scope load: served by the L2, never by a possibly stale vector L1
+// line (words written by other waves within the same kernel).
+__device__ __forceinline__ uint32_t ld_agent(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // e / L for e < 2^32 with magic = min(floor(2^32 / L), 2^32 - 1): the
 // multiply-high estimate is at most one below the quotient.
 __device__ __forceinline__ uint32_t div_L(uint32_t e, uint32_t L, uint32_t magic) {
@@ -236,6 +253,99 @@ __device__ __forceinline__ void block_defer_push(const Inst& in, uint64_t* list,
 }
 
 // ----------------------------------------------------------- graph set-up
+// Set-up words read back by the host (one allocation, zeroed first).
+enum Prep : int { kPrepMaxDeg = 0, kPrepBad = 1, kPrepDefer = 2, kPrepHeavy = 3, kPrepMaxNz = 4, kPrepWords = 8 };
+enum PrepBad : unsigned long long { kBadRowPtr = 1, kBadCol = 2 };
+
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long x) {
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        const unsigned long long y = __shfl_xor(x, o, kWave);
+        x = y > x ? y : x;
+    }
+    return x;
+}
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long x) {
+    for (int o = kWave / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, kWave);
+    return x;
+}
+
+// 32-bit caller offsets widened to int64.
+__global__ void k_widen_offsets(const int32_t* src, uint64_t count, int64_t* dst) {
+    for (uint64_t t = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; t < count; t += uint64_t(gridDim.x) * blockDim.x)
+        dst[t] = src[t];
+}
+
+// Per vertex v (caller's ids): tot[v] = entries over all layers, nz[v] =
+// layers in which v has an edge, ids[v] = v (the sort's values); the
+// largest layer degree and a flag for decreasing offsets go to prep.
+__global__ void k_prep_vertex(const int64_t* const* row_ptr, uint32_t n, uint32_t L, uint64_t* tot, uint8_t* nz,
+                              uint32_t* ids, unsigned long long* prep) {
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    const uint64_t first = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+    // every lane runs the same number of rounds so the wave reductions see all lanes
+    for (uint64_t base = first - threadIdx.x % kWave; base < n; base += stride) {
+        const uint64_t v = base + threadIdx.x % kWave;
+        unsigned long long md = 0, bad = 0;
+        if (v < n) {
+            uint64_t t = 0;
+            uint32_t z = 0;
+            for (uint32_t l = 0; l < L; ++l) {
+                const int64_t d = row_ptr[l][v + 1] - row_ptr[l][v];
+                if (d < 0) {
+                    bad = kBadRowPtr;
+                    continue;
+                }
+                t += uint64_t(d);
+                z += d > 0;
+                md = uint64_t(d) > md ? uint64_t(d) : md;
+            }
+            tot[v] = t;
+            nz[v] = uint8_t(z);
+            ids[v] = uint32_t(v);
+        }
+        md = wave_max_u64(md);
+        bad = wave_max_u64(bad);
+        if (lane_id() == 0) {
+            if (md) atomicMax(&prep[kPrepMaxDeg], md);
+            if (bad) atomicOr(&prep[kPrepBad], bad);
+        }
+    }
+}
+
+// Rank r of the degree order (tot_s descending, perm[r] = caller's id):
+// inverse permutation, z in internal ids, chunk counts of the static chunk
+// list; the deferred-chunk bound, the heavy head (ranks with more than
+// kPullHeavy entries) and the largest z go to prep.
+template <uint64_t kChunkEntries, uint64_t kDeferEntries, uint64_t kHeavyEntries>
+__global__ void k_prep_rank(const uint64_t* tot_s, const uint32_t* perm, const uint8_t* nz, uint32_t n, uint32_t* inv,
+                            uint8_t* zr, uint64_t* chunk_cnt, unsigned long long* prep) {
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    const uint64_t first = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+    for (uint64_t base = first - threadIdx.x % kWave; base < n; base += stride) {
+        const uint64_t r = base + threadIdx.x % kWave;
+        unsigned long long defer = 0, heavy = 0, mz = 0;
+        if (r < n) {
+            const uint32_t v = perm[r];
+            const uint64_t t = tot_s[r];
+            inv[v] = uint32_t(r);
+            const uint8_t z = nz[v];
+            zr[r] = z;
+            mz = z;
+            chunk_cnt[r] = (t + kChunkEntries - 1) / kChunkEntries;
+            defer = (t + kDeferEntries - 1) / kDeferEntries;
+            if (t > kHeavyEntries) heavy = r + 1;  // totals descend with r
+        }
+        defer = wave_sum_u64(defer);
+        heavy = wave_max_u64(heavy);
+        mz = wave_max_u64(mz);
+        if (lane_id() == 0) {
+            if (defer) atomicAdd(&prep[kPrepDefer], defer);
+            if (heavy) atomicMax(&prep[kPrepHeavy], heavy);
+            if (mz) atomicMax(&prep[kPrepMaxNz], mz);
+        }
+    }
+}
+
 // deg0[v][l] = deg_l(perm[v]) from per-layer offsets (v: internal id,
 // perm[v]: caller's id).
 __global__ void k_build_deg0(const int64_t* const* row_ptr, const uint32_t* perm, uint32_t n, uint32_t L,
@@ -248,13 +358,16 @@ __global__ void k_build_deg0(const int64_t* const* row_ptr, const uint32_t* perm
 }
 
 // Merged vertex-major adjacency in internal ids, and the target's z per
-// entry; one wavefront per vertex.
+// entry; one wavefront per vertex.  A neighbour id >= n sets kBadCol in
+// prep[kPrepBad] (the entry is written as a self-reference of no weight:
+// target 0, z 0) so the host can reject the graph.
 __global__ void k_build_adj(const int64_t* const* row_ptr, const uint32_t* const* col, const uint32_t* deg0,
                             const int64_t* off, const uint32_t* perm, const uint32_t* inv, const uint8_t* z_caller,
-                            uint32_t n, uint32_t L, uint32_t* adj, uint8_t* adjz) {
+                            uint32_t n, uint32_t L, uint32_t* adj, uint8_t* adjz, unsigned long long* prep) {
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
     const uint32_t nwaves = gridDim.x * blockDim.x / kWave;
     const uint32_t lane = lane_id();
+    bool bad = false;
     for (uint32_t v = wave; v < n; v += nwaves) {
         const uint32_t ov = perm[v];
         uint64_t dst = uint64_t(off[v]);
@@ -263,12 +376,16 @@ __global__ void k_build_adj(const int64_t* const* row_ptr, const uint32_t* const
             const uint32_t d = deg0[uint64_t(v) * L + l];
             const uint32_t* src = col[l] + s;
             for (uint32_t i = lane; i < d; i += kWave) {
-                adj[dst + i] = inv[src[i]] * L + l;
-                adjz[dst + i] = z_caller[src[i]];
+                const uint32_t c = src[i];
+                const bool ok = c < n;
+                bad |= !ok;
+                adj[dst + i] = ok ? inv[c] * L + l : 0u;
+                adjz[dst + i] = ok ? z_caller[c] : uint8_t(0);
             }
             dst += d;
         }
     }
+    if (__ballot(bad) && lane == 0 && prep) atomicOr(&prep[kPrepBad], (unsigned long long)kBadCol);
 }
 
 // out[i][w] = core index of instance i at internal id inv[w]: results in
@@ -822,7 +939,7 @@ __device__ __forceinline__ void cascade_expand(const Inst& in, InstState* st, Co
         for (uint32_t r = 0; r < kUnroll; ++r) emit_removed(in, st, ctl, sh, ring, next_parity, head, rem[r], u[r]);
         if (agg) agg_flush(in, st, ctl, sh, ring, next_parity, head, thr, k);
     }
-    __syncthreads();
+    sync_stores();  // ring entries written above are read by other waves next
 }
 
 // Cascade of every instance not done: the chunks of this step's lists, then
@@ -885,7 +1002,7 @@ __global__ void __launch_bounds__(kBlock) k_cascade(const Inst* insts, InstState
             uint64_t start = 0;
             uint32_t len = 0;
             if (threadIdx.x < nb) {
-                const uint32_t v = ring[(head + threadIdx.x) % in.cap];
+                const uint32_t v = ld_agent(&ring[(head + threadIdx.x) % in.cap]);
                 start = uint64_t(in.off[v]);
                 len = uint32_t(uint64_t(in.off[v + 1]) - start);
             }

@@ -5,6 +5,7 @@ launch."""
 import ctypes as C
 import dataclasses
 import sys
+import threading
 
 import numpy as np
 import pytest
@@ -131,3 +132,43 @@ def test_batch_errors():
         b.decompose()
         out = np.empty(16, dtype=np.int32)
         assert lib().hg_batch_core_index(b._h, len(gs), out.ctypes.data_as(C.POINTER(C.c_int32))) == -1
+
+
+def test_batch_rejects_bad_neighbour_id():
+    from types import SimpleNamespace as NS
+    good = G.graph(4, [[(0, 1), (2, 3)]])
+    bad = NS(n=3, L=1, GetN=lambda: 3, layers=lambda: [NS(row_ptr=np.array([0, 1, 2, 2], dtype=np.int64),
+                                                         col=np.array([5, 0], dtype=np.uint32))])
+    with pytest.raises(HgError) as e:
+        Batch([good, bad])
+    assert e.value.code == -1
+
+
+def test_concurrent_batches_from_threads():
+    """Batches launched from four host threads at once (their persistent
+    kernels share the GPU): every graph still gets the oracle's indices.
+    Waves of one workgroup hand FIFO entries and compacted lists to each
+    other through global memory; under concurrent kernels a store can still
+    be in flight at a plain barrier, which this load exposes."""
+    pool = [MultilayerGraph.generate(batch_spec(g, base=0xBD, n_lo=300, n_hi=8000)) for g in range(24)]
+    refs = [O.coreindex(mg.n, mg.layers()) for mg in pool]
+    say("case concurrent batches from 4 threads")
+    bad = []
+
+    def worker(w):
+        rng = np.random.default_rng(1000 + w)
+        for t in range(30):
+            sel = [int(x) for x in rng.integers(0, len(pool), size=int(rng.integers(1, 9)))]
+            with Batch([pool[i] for i in sel]) as b:
+                b.set_params(block=512, unroll=4, blocks_per_cu=2, order=0, use_graph=0)
+                b.decompose()
+                for j, i in enumerate(sel):
+                    if not np.array_equal(b.core_index(j), refs[i]):
+                        bad.append((w, t, i))
+
+    ts = [threading.Thread(target=worker, args=(w,)) for w in range(4)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    assert not bad, bad

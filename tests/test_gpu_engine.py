@@ -183,6 +183,34 @@ def test_errors():
             eng.densest(1.0)
 
 
+def test_invalid_graphs_rejected():
+    """Offsets that decrease and neighbour ids >= n are caught by the device
+    set-up and reported as argument errors, not decomposed."""
+    say("case invalid graphs")
+    from types import SimpleNamespace as NS
+    good = NS(row_ptr=np.array([0, 1, 2, 2], dtype=np.int64), col=np.array([1, 0], dtype=np.uint32))
+    down = NS(row_ptr=np.array([0, 2, 1, 2], dtype=np.int64), col=np.array([1, 0], dtype=np.uint32))
+    wide = NS(row_ptr=np.array([0, 1, 2, 2], dtype=np.int64), col=np.array([7, 0], dtype=np.uint32))
+    with Engine([good], 3) as eng:
+        eng.decompose()
+        assert eng.core_index().tolist() == [[1, 1, 0]]
+    for bad in (down, wide):
+        with pytest.raises(HgError) as e:
+            Engine([good, bad], 3)
+        assert e.value.code == -1
+
+
+@pytest.mark.parametrize("seed", [11, 12])
+def test_int32_offsets(seed):
+    """32-bit caller offsets (widened on the device) give the same results."""
+    mg = MultilayerGraph.generate(small_spec(4000, 5, 9000, seed))
+    say(f"case int32 offsets seed={seed}")
+    with Engine(mg.layers(), mg.n, offset_bits=32) as eng:
+        eng.decompose()
+        ci = eng.core_index()
+    assert np.array_equal(ci, O.coreindex(mg.n, mg.layers()))
+
+
 def test_wds_coreindex_float_counter_saturation():
     """A 4200-clique in two identical layers: every cell is the whole clique
     with 4200 * 4199 > 2^24 directed entries per layer, so the reference's
