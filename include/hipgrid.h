@@ -226,12 +226,15 @@ int hg_batch_destroy(hg_batch* b);
  * "densest") are copied in at submit and queue in arrival order; `lanes`
  * worker threads, each with its own HIP stream, take work when idle: a
  * request with more than `large_entries` directed entries runs alone on the
- * single-graph engine (hg_graph_*), the others are taken from the head of
- * the queue, up to `max_batch` at a time, and run as one batch (hg_batch_*).
- * A request is admitted only while the device bytes of all work in flight
- * stay within `hbm_budget`.  Requests that arrive while others run join the
- * next batch a lane forms; the lanes' work overlaps on the device.  Every
- * answer equals that of hg_graph_* on the request's graph alone.
+ * single-graph engine (hg_graph_*).  When at least `min_batch` small
+ * requests wait at the head of the queue, a lane takes up to `max_batch` of
+ * them and runs them as one batch (hg_batch_*, the throughput form); with
+ * fewer waiting it takes one and runs it alone on the single-graph engine
+ * (the latency form: the GPU is not saturated).  A request is admitted only
+ * while the device bytes of all work in flight stay within `hbm_budget`.
+ * Requests that arrive while others run join the next batch a lane forms;
+ * the lanes' work overlaps on the device.  Every answer equals that of
+ * hg_graph_* on the request's graph alone.
  * plain = 1 is the one-at-a-time baseline: one lane, every request alone on
  * the single-graph engine.  The reference has no serving layer (main.cpp
  * runs one method on one graph per process). */
@@ -243,7 +246,7 @@ typedef struct hg_server_config {
     int32_t lanes;          /* concurrent workers (streams), 1..16 */
     int32_t max_batch;      /* requests per batch, >= 1 */
     int32_t plain;          /* 1: one request at a time on the single-graph engine */
-    int32_t reserved;
+    int32_t min_batch;      /* small requests waiting before a lane batches them (0: 8) */
     hg_batch_params params; /* launch shape of the batches (block == 0: defaults) */
 } hg_server_config;
 

@@ -12,6 +12,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "hipgrid.h"
@@ -39,6 +40,38 @@ hipError_t dalloc(T** p, size_t count) {
     return hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T));
 }
 
+// Runs fn(begin, end) over [0, count) in contiguous ranges of at least
+// min_per_thread items on up to 16 host threads (the caller's included).
+template <typename F>
+void par_ranges(size_t count, size_t min_per_thread, F&& fn) {
+    size_t T = std::min<size_t>(std::max(1u, std::thread::hardware_concurrency()), 16);
+    T = std::min<size_t>(T, std::max<size_t>(1, count / std::max<size_t>(min_per_thread, 1)));
+    if (T <= 1) {
+        fn(size_t(0), count);
+        return;
+    }
+    const size_t step = (count + T - 1) / T;
+    std::vector<std::thread> th;
+    th.reserve(T - 1);
+    for (size_t t = 1; t < T; ++t) {
+        const size_t b = t * step, e = std::min(count, b + step);
+        if (b >= e) break;
+        try {
+            th.emplace_back([&fn, b, e] { fn(b, e); });
+        } catch (...) {  // no thread available: run the range here
+            fn(b, e);
+        }
+    }
+    fn(size_t(0), std::min(count, step));
+    for (auto& x : th) x.join();
+}
+
+inline void par_memcpy(void* dst, const void* src, size_t bytes) {
+    par_ranges(bytes, size_t(2) << 20, [&](size_t b, size_t e) {
+        std::memcpy(static_cast<char*>(dst) + b, static_cast<const char*>(src) + b, e - b);
+    });
+}
+
 // Every copy between caller (pageable) memory and the device goes through
 // pinned buffers the library owns: the host side is a plain memcpy into or
 // out of a pinned slot, the device side a DMA on the handle's stream.  A
@@ -49,7 +82,7 @@ hipError_t dalloc(T** p, size_t count) {
 // is copying at that moment).
 struct SlotSet {
     static constexpr int kSlots = 4;
-    static constexpr size_t kSlotBytes = size_t(8) << 20;
+    static constexpr size_t kSlotBytes = size_t(16) << 20;
     int device = 0;
     char* buf[kSlots] = {};
     hipEvent_t ev[kSlots] = {};
@@ -85,7 +118,7 @@ public:
             const size_t len = std::min(SlotSet::kSlotBytes, bytes - off);
             const int s = take();
             if ((e = settle(s)) != hipSuccess) break;
-            std::memcpy(set_->buf[s], static_cast<const char*>(src) + off, len);
+            par_memcpy(set_->buf[s], static_cast<const char*>(src) + off, len);
             e = hipMemcpyAsync(static_cast<char*>(dst) + off, set_->buf[s], len, hipMemcpyHostToDevice, stream_);
             if (e == hipSuccess) e = hipEventRecord(set_->ev[s], stream_);
             busy_[s] = e == hipSuccess;
@@ -157,7 +190,7 @@ private:
         hipError_t e = hipSuccess;
         if (busy_[s]) e = hipEventSynchronize(set_->ev[s]);
         busy_[s] = false;
-        if (out_[s] && e == hipSuccess) std::memcpy(out_[s], set_->buf[s], out_len_[s]);
+        if (out_[s] && e == hipSuccess) par_memcpy(out_[s], set_->buf[s], out_len_[s]);
         out_[s] = nullptr;
         return e;
     }

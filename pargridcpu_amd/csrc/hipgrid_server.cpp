@@ -2,15 +2,19 @@
 //
 // Requests (a graph plus "decompose" or "densest") are copied into the
 // server at submit and queue in arrival order.  `lanes` worker threads, each
-// with its own HIP stream, take work whenever they are idle: a request with
+// with its own HIP stream, take work whenever they are idle.  A request with
 // more than `large_entries` directed entries runs alone on the single-graph
-// engine (hg_graph_*, the whole GPU per level); otherwise the lane takes the
-// queue's leading small requests, up to `max_batch` and as many as the
-// device-memory budget admits, and runs them as one batch (hg_batch_*: one
-// persistent launch).  Requests that arrive while batches are on the GPU
-// join the next batch a lane forms, and the lanes' uploads, launches and
-// copies overlap on the device.  A request is admitted only while the bytes
-// of everything in flight stay within `hbm_budget`.
+// engine (hg_graph_*, the whole GPU per level).  Small requests are batched
+// only when the queue holds at least `min_batch` of them at its head: then
+// the lane takes them, up to `max_batch` and as many as the device-memory
+// budget admits, and runs them as one batch (hg_batch_*: one persistent
+// launch, each (graph, lmd) chain on one workgroup -- the throughput form).
+// With fewer waiting, the GPU is not saturated and a request finishes
+// sooner on the single-graph engine, so the lane takes one and runs it
+// alone (the latency form).  Requests that arrive while others run join the
+// next batch a lane forms, and the lanes' uploads, launches and copies
+// overlap on the device.  A request is admitted only while the bytes of
+// everything in flight stay within `hbm_budget`.
 //
 // plain = 1 is the baseline: one lane, one request at a time, each on the
 // single-graph engine.
@@ -21,6 +25,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -49,12 +54,13 @@ struct Request {
     uint64_t D = 0;
     uint64_t maxdeg = 0;
     int64_t est_bytes = 0;
-    std::vector<std::vector<int64_t>> rp;
-    std::vector<std::vector<uint32_t>> col;
+    // the graph, copied at submit (buffers left uninitialised before the copy)
+    std::vector<std::unique_ptr<int64_t[]>> rp;
+    std::vector<std::unique_ptr<uint32_t[]>> col;
     // result
     int32_t status = 1;  // 1 pending, 0 done, < 0 error
     std::string err;
-    std::vector<int32_t> core_index;
+    std::unique_ptr<int32_t[]> core_index;  // [L][n]
     std::vector<int32_t> kmax;
     hg_wds wds{};
     int32_t batch_size = 0;
@@ -66,20 +72,26 @@ double ms_between(Clock::time_point a, Clock::time_point b) {
 }
 
 // Device bytes a request holds while it runs (graph, per-lmd state,
-// upload temporaries), on the engine that will run it.
-int64_t estimate_bytes(const Request& r, bool large) {
+// set-up temporaries) on the single-graph engine and in a batch.
+int64_t engine_bytes(const Request& r) {
+    const uint64_t n = uint64_t(r.n), L = uint64_t(r.L), D = r.D;
+    const uint64_t setup = 8 * L * (n + 1) + 4 * D + 64 * (n + 1) + 16 * n;  // CSR, sort and scan buffers
+    const uint64_t graph = 8 * (n + 1) + 5 * D + 4 * n * L + 5 * n + 4 * n * L + D / 8 + 8;
+    const uint64_t state = L * (4 * n * L + 8 * n + 8 * n + n / 8 + 16 * (D / 4096 + n + 1));
+    const uint64_t dens = 8 * L * (r.maxdeg + 2) * 4;
+    return int64_t(graph + state + dens + setup);
+}
+int64_t batch_bytes(const Request& r) {
     const uint64_t n = uint64_t(r.n), L = uint64_t(r.L), D = r.D;
     const uint64_t upload = 8 * L * (n + 1) + 4 * D + 8 * (n + 1) + 5 * n;
-    if (large) {
-        const uint64_t graph = 8 * (n + 1) + 5 * D + 4 * n * L + 5 * n + 4 * n * L + D / 8 + 8;
-        const uint64_t state = L * (4 * n * L + 8 * n + 8 * n + n / 8 + 16 * (D / 4096 + n + 1));
-        const uint64_t dens = 8 * L * (r.maxdeg + 2) * 4;
-        return int64_t(graph + state + dens + upload);
-    }
     const uint64_t graph = 4 * (n + 1) + 5 * D + 5 * n + 4 * n * L + 4 * n * L;
     const uint64_t state = L * (4 * n * L + 16 * n + 4 * (n / 32 + 1));
     const uint64_t dens = r.kind == HG_REQ_DENSEST ? 4 * L * L * (r.maxdeg + 2) : 0;
     return int64_t(graph + state + dens + upload);
+}
+// Large requests always run on the engine; small ones may run either way.
+int64_t estimate_bytes(const Request& r, bool large) {
+    return large ? engine_bytes(r) : std::max(engine_bytes(r), batch_bytes(r));
 }
 
 }  // namespace
@@ -115,8 +127,6 @@ void finish(hg_server* s, std::vector<Request*>& rs, int rc, const std::string& 
         r->err = err;
         r->rp.clear();
         r->col.clear();
-        r->rp.shrink_to_fit();
-        r->col.shrink_to_fit();
         if (rc == HG_OK) {
             ++s->completed;
             s->latencies.push_back(ms_between(r->t_submit, now));
@@ -130,16 +140,16 @@ void finish(hg_server* s, std::vector<Request*>& rs, int rc, const std::string& 
     s->cv_work.notify_all();
 }
 
-int run_large(Request* r, hipStream_t stream) {
+int run_alone(Request* r, hipStream_t stream) {
     std::vector<hg_layer_csr> lay(size_t(r->L));
-    for (int l = 0; l < r->L; ++l) lay[size_t(l)] = {r->rp[size_t(l)].data(), r->col[size_t(l)].data()};
+    for (int l = 0; l < r->L; ++l) lay[size_t(l)] = {r->rp[size_t(l)].get(), r->col[size_t(l)].get()};
     hg_graph* g = nullptr;
     int rc = hg_graph_create(lay.data(), r->L, r->n, 64, stream, &g);
     if (rc) return rc;
     rc = hg_decompose(g);
     if (!rc) {
-        r->core_index.resize(size_t(r->n) * size_t(r->L));
-        rc = hg_core_index(g, r->core_index.data());
+        r->core_index.reset(new int32_t[size_t(r->n) * size_t(r->L)]);
+        rc = hg_core_index(g, r->core_index.get());
     }
     if (!rc) {
         r->kmax.resize(size_t(r->L));
@@ -156,7 +166,7 @@ int run_batch(hg_server* s, std::vector<Request*>& rs, hipStream_t stream) {
     for (size_t i = 0; i < rs.size(); ++i) {
         Request* r = rs[i];
         lays[i].resize(size_t(r->L));
-        for (int l = 0; l < r->L; ++l) lays[i][size_t(l)] = {r->rp[size_t(l)].data(), r->col[size_t(l)].data()};
+        for (int l = 0; l < r->L; ++l) lays[i][size_t(l)] = {r->rp[size_t(l)].get(), r->col[size_t(l)].get()};
         gs[i].layers = lays[i].data();
         gs[i].L = r->L;
         gs[i].offset_bits = 64;
@@ -177,8 +187,8 @@ int run_batch(hg_server* s, std::vector<Request*>& rs, hipStream_t stream) {
     std::vector<double> betas;
     for (size_t i = 0; i < rs.size() && !rc; ++i) {
         Request* r = rs[i];
-        r->core_index.resize(size_t(r->n) * size_t(r->L));
-        rc = hg_batch_core_index(b, int32_t(i), r->core_index.data());
+        r->core_index.reset(new int32_t[size_t(r->n) * size_t(r->L)]);
+        rc = hg_batch_core_index(b, int32_t(i), r->core_index.get());
         if (!rc) {
             r->kmax.resize(size_t(r->L));
             rc = hg_batch_cells(b, int32_t(i), nullptr, 0, r->kmax.data());
@@ -206,7 +216,7 @@ void lane_main(hg_server* s) {
     for (;;) {
         std::vector<Request*> take;
         int64_t bytes = 0;
-        bool large = false;
+        bool large = false, alone = false;
         {
             std::unique_lock<std::mutex> lk(s->mu);
             for (;;) {
@@ -232,11 +242,18 @@ void lane_main(hg_server* s) {
             }
             Request* first = s->pending.front();
             large = is_large(s, first);
-            const int32_t cap = large ? 1 : std::max(1, s->cfg.max_batch);
+            if (!large) {  // batch only a queue of at least min_batch small requests
+                const int32_t want = std::min(s->cfg.min_batch, std::max(1, s->cfg.max_batch));
+                int32_t small = 0;
+                for (auto it = s->pending.begin(); it != s->pending.end() && small < want && !is_large(s, *it); ++it)
+                    ++small;
+                alone = small < want;
+            }
+            const int32_t cap = large || alone ? 1 : std::max(1, s->cfg.max_batch);
             while (!s->pending.empty() && int32_t(take.size()) < cap) {
                 Request* r = s->pending.front();
                 if (!take.empty() && is_large(s, r)) break;  // a large one runs alone, in arrival order
-                if (s->inflight + bytes + r->est_bytes > s->cfg.hbm_budget) break;
+                if (s->inflight + bytes + r->est_bytes > s->cfg.hbm_budget) break;  // est covers both engines
                 s->pending.pop_front();
                 take.push_back(r);
                 bytes += r->est_bytes;
@@ -251,7 +268,7 @@ void lane_main(hg_server* s) {
         }
         int rc;
         try {
-            rc = large ? run_large(take[0], stream) : run_batch(s, take, stream);
+            rc = large || alone ? run_alone(take[0], stream) : run_batch(s, take, stream);
         } catch (const std::bad_alloc&) {
             rc = fail(HG_ERR_NOMEM, "host allocation failed");
         }
@@ -281,6 +298,8 @@ int hg_server_create(const hg_server_config* cfg, hg_server** out) {
     std::unique_ptr<hg_server> s(new (std::nothrow) hg_server());
     if (!s) return fail(HG_ERR_NOMEM, "host allocation failed");
     s->cfg = *cfg;
+    if (s->cfg.min_batch < 0) return fail(HG_ERR_ARG, "min_batch must be >= 0");
+    if (s->cfg.min_batch == 0) s->cfg.min_batch = 8;
     if (s->cfg.plain) {
         s->cfg.lanes = 1;
         s->cfg.max_batch = 1;
@@ -322,22 +341,39 @@ int hg_server_submit(hg_server* s, const hg_batch_graph* graph, int32_t kind, do
         for (int l = 0; l < graph->L; ++l) {
             const hg_layer_csr& lay = graph->layers[l];
             if (!lay.row_ptr || !lay.col) return fail(HG_ERR_ARG, "layer pointer is NULL");
-            std::vector<int64_t>& rp = r->rp[size_t(l)];
-            rp.resize(n + 1);
-            if (graph->offset_bits == 64) {
-                std::memcpy(rp.data(), lay.row_ptr, 8 * (n + 1));
-            } else {
-                const int32_t* src = static_cast<const int32_t*>(lay.row_ptr);
-                for (size_t v = 0; v <= n; ++v) rp[v] = src[v];
-            }
+            int64_t* rp = new int64_t[n + 1];
+            r->rp[size_t(l)].reset(rp);
+            // copy (widening 32-bit offsets), then check, each in parallel ranges
+            hgc::par_ranges(n + 1, size_t(1) << 20, [&](size_t b, size_t e) {
+                if (graph->offset_bits == 64) {
+                    std::memcpy(rp + b, static_cast<const int64_t*>(lay.row_ptr) + b, 8 * (e - b));
+                } else {
+                    const int32_t* src = static_cast<const int32_t*>(lay.row_ptr);
+                    for (size_t v = b; v < e; ++v) rp[v] = src[v];
+                }
+            });
+            std::atomic<bool> bad{false};
+            std::atomic<uint64_t> maxdeg{r->maxdeg};
+            hgc::par_ranges(n, size_t(1) << 20, [&](size_t b, size_t e) {
+                uint64_t md = 0;
+                bool bd = false;
+                for (size_t v = b; v < e; ++v) {
+                    bd |= rp[v + 1] < rp[v];
+                    md = std::max<uint64_t>(md, uint64_t(rp[v + 1] - rp[v]));
+                }
+                if (bd) bad = true;
+                uint64_t cur = maxdeg.load();
+                while (md > cur && !maxdeg.compare_exchange_weak(cur, md)) {
+                }
+            });
             if (rp[0] != 0) return fail(HG_ERR_ARG, "row_ptr[0] must be 0");
-            for (size_t v = 0; v < n; ++v) {
-                if (rp[v + 1] < rp[v]) return fail(HG_ERR_ARG, "row_ptr must be non-decreasing");
-                r->maxdeg = std::max<uint64_t>(r->maxdeg, uint64_t(rp[v + 1] - rp[v]));
-            }
+            if (bad) return fail(HG_ERR_ARG, "row_ptr must be non-decreasing");
+            r->maxdeg = maxdeg.load();
             const size_t m = size_t(rp[n]);
-            r->col[size_t(l)].assign(lay.col, lay.col + m);
-            if (m == 0) r->col[size_t(l)].push_back(0);  // a non-NULL pointer for an empty layer
+            uint32_t* col = new uint32_t[std::max<size_t>(m, 1)];  // a non-NULL pointer for an empty layer
+            r->col[size_t(l)].reset(col);
+            col[0] = 0;
+            if (m) hgc::par_memcpy(col, lay.col, 4 * m);
             r->D += m;
         }
     } catch (const std::bad_alloc&) {
@@ -389,7 +425,7 @@ int hg_server_result(hg_server* s, int64_t id, int32_t* core_index, int32_t* kma
     Request* r = it->second.get();
     if (r->status == 1) return fail(HG_ERR_STATE, "request still pending");
     if (r->status < 0) return fail(r->status, r->err);
-    if (core_index) std::memcpy(core_index, r->core_index.data(), 4 * r->core_index.size());
+    if (core_index && r->core_index) hgc::par_memcpy(core_index, r->core_index.get(), 4 * size_t(r->n) * size_t(r->L));
     if (kmax) std::memcpy(kmax, r->kmax.data(), 4 * r->kmax.size());
     return HG_OK;
 }

@@ -63,7 +63,7 @@ class BatchParams(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget", C.c_int64), ("large_entries", C.c_int64), ("lanes", C.c_int32), ("max_batch", C.c_int32),
-                ("plain", C.c_int32), ("reserved", C.c_int32), ("params", BatchParams)]
+                ("plain", C.c_int32), ("min_batch", C.c_int32), ("params", BatchParams)]
 
 
 class RequestInfo(C.Structure):
@@ -426,13 +426,15 @@ class Server:
     decomposition or densest-subgraph search, wait for their answers."""
 
     def __init__(self, hbm_budget: int = 200 << 30, lanes: int = 4, max_batch: int = 64,
-                 large_entries: int = 50_000_000, plain: bool = False, params: dict | None = None):
+                 large_entries: int = 50_000_000, plain: bool = False, params: dict | None = None,
+                 min_batch: int = 0):
         cfg = ServerConfig()
         cfg.hbm_budget = int(hbm_budget)
         cfg.large_entries = int(large_entries)
         cfg.lanes = int(lanes)
         cfg.max_batch = int(max_batch)
         cfg.plain = 1 if plain else 0
+        cfg.min_batch = int(min_batch)
         for k, v in (params or {}).items():
             setattr(cfg.params, k, int(v))
         h = _P()

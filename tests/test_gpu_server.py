@@ -50,16 +50,18 @@ def check(srv, rid, mg, kind, beta):
     return info
 
 
-@pytest.mark.parametrize("lanes,max_batch,plain", [(3, 8, False), (1, 64, False), (1, 1, True)])
-def test_mix_equals_alone(lanes, max_batch, plain):
+@pytest.mark.parametrize("lanes,max_batch,plain,min_batch", [(3, 8, False, 1), (1, 64, False, 0), (1, 1, True, 0),
+                                                               (4, 16, False, 4)])
+def test_mix_equals_alone(lanes, max_batch, plain, min_batch):
     gs = request_graphs(20, 0xD1)
     kinds = [REQ_DENSEST if i % 5 == 0 else REQ_DECOMPOSE for i in range(len(gs))]
     betas = [1.0 if i % 2 else 0.5 for i in range(len(gs))]
-    say(f"case server lanes={lanes} max_batch={max_batch} plain={plain}")
+    say(f"case server lanes={lanes} max_batch={max_batch} plain={plain} min_batch={min_batch}")
     # large_entries low enough that the two largest graphs run alone on the
-    # single-graph engine while the rest are batched
+    # single-graph engine while the rest are batched (min_batch 1: whenever
+    # any small request waits) or run alone when few wait
     big = sorted(g.directed_entries() for g in gs)[-3]
-    with Server(lanes=lanes, max_batch=max_batch, plain=plain, large_entries=big) as srv:
+    with Server(lanes=lanes, max_batch=max_batch, plain=plain, large_entries=big, min_batch=min_batch) as srv:
         ids = []
         for i, g in enumerate(gs):  # staggered arrivals: later ones join batches formed meanwhile
             ids.append(srv.submit(g, kinds[i], betas[i]))
