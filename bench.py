@@ -164,13 +164,16 @@ def run_batched(steps: int, count: int = 256, params: str = "tune") -> dict:
             "exact": bool(exact), "params": best, "roofline": roof}
 
 
-def run_serving(n_requests: int = 160, rate: float = 20.0) -> dict:
-    """Serving form: Poisson arrivals at `rate` requests/s, 80% decompositions
-    and 20% densest-subgraph searches; graphs from the batched workload's
+def run_serving(n_requests: int = 160, rates=(20.0, 200.0)) -> dict:
+    """Serving form: Poisson arrivals, 80% decompositions and 20%
+    densest-subgraph searches; graphs from the batched workload's
     distribution, 10% of them large (n = 5M, L = 12, 60M sampled edges);
-    200 GB device-memory budget.  The request loop (4 lanes, continuous
-    batching) against one-at-a-time execution on the single-graph engine,
-    same arrival trace; every answer must be identical in both."""
+    200 GB device-memory budget.  The request loop (4 lanes: small requests
+    batched when 8 or more wait, else run alone on the single-graph engine)
+    against one-at-a-time execution on the single-graph engine, same arrival
+    trace, at each offered rate (20 req/s: the GPU is not saturated; 200
+    req/s: arrivals outpace one-at-a-time service and the queue builds up).
+    Every answer must be identical in both."""
     import math
 
     import numpy as np
@@ -182,15 +185,19 @@ def run_serving(n_requests: int = 160, rate: float = 20.0) -> dict:
     small = [MultilayerGraph.generate(batch_spec(r, base=0xC5)) for r in range(48)]
     large = [MultilayerGraph.generate(large_request_spec(r)) for r in range(2)]
     log(f"[bench] serving: generated the request graphs in {time.perf_counter() - t0:.1f}s")
-    trace, t = [], 0.0
-    for r in range(n_requests):
-        seed = (0xC5 << 32) | r
-        kind = REQ_DENSEST if _unit(seed, 10) < 0.2 else REQ_DECOMPOSE
-        g = large[int(_unit(seed, 12) * len(large))] if _unit(seed, 11) < 0.1 else small[int(_unit(seed, 13) * len(small))]
-        t += -math.log(1.0 - _unit(seed, 14)) / rate
-        trace.append((t, g, kind))
 
-    def serve(**kw):
+    def make_trace(rate):
+        trace, t = [], 0.0
+        for r in range(n_requests):
+            seed = (0xC5 << 32) | r
+            kind = REQ_DENSEST if _unit(seed, 10) < 0.2 else REQ_DECOMPOSE
+            g = (large[int(_unit(seed, 12) * len(large))] if _unit(seed, 11) < 0.1
+                 else small[int(_unit(seed, 13) * len(small))])
+            t += -math.log(1.0 - _unit(seed, 14)) / rate
+            trace.append((t, g, kind))
+        return trace
+
+    def serve(trace, **kw):
         with Server(hbm_budget=200 << 30, **kw) as srv:
             start = time.perf_counter()
             ids = []
@@ -205,24 +212,29 @@ def run_serving(n_requests: int = 160, rate: float = 20.0) -> dict:
             outs = [srv.result(i)[0] for i in ids]
         return st, infos, outs
 
-    res = {}
-    outs0 = None
-    for name, kw in (("continuous", dict(lanes=4, max_batch=64, large_entries=20_000_000)), ("one_at_a_time", dict(plain=True))):
-        st, infos, outs = serve(**kw)
-        sizes = [i.batch_size for i in infos]
-        res[name] = {"p50_ms": round(st["p50_ms"], 2), "p99_ms": round(st["p99_ms"], 2),
-                     "req_per_s": round(n_requests / st["span_s"], 2), "completed": st["completed"],
-                     "failed": st["failed"], "batches": st["batches"], "mean_batch": round(sum(sizes) / len(sizes), 2)}
-        log(f"[bench] serving {name}: {res[name]}")
-        if outs0 is None:
-            outs0 = outs
-        else:
-            res["identical"] = all(np.array_equal(a, c) for a, c in zip(outs0, outs))
-    res["requests"] = n_requests
-    res["offered_rate"] = rate
-    c, p = res["continuous"], res["one_at_a_time"]
-    res["p99_ratio"] = round(p["p99_ms"] / c["p99_ms"], 2) if c["p99_ms"] else None
-    return res
+    by_rate = []
+    for rate in rates:
+        trace = make_trace(rate)
+        res = {"offered_rate": rate}
+        outs0 = None
+        for name, kw in (("continuous", dict(lanes=4, max_batch=64, large_entries=20_000_000)),
+                         ("one_at_a_time", dict(plain=True))):
+            st, infos, outs = serve(trace, **kw)
+            sizes = [i.batch_size for i in infos]
+            res[name] = {"p50_ms": round(st["p50_ms"], 2), "p99_ms": round(st["p99_ms"], 2),
+                         "req_per_s": round(n_requests / st["span_s"], 2), "completed": st["completed"],
+                         "failed": st["failed"], "batches": st["batches"],
+                         "mean_batch": round(sum(sizes) / len(sizes), 2)}
+            log(f"[bench] serving {rate:g} req/s {name}: {res[name]}")
+            if outs0 is None:
+                outs0 = outs
+            else:
+                res["identical"] = all(np.array_equal(a, c) for a, c in zip(outs0, outs))
+        c, p = res["continuous"], res["one_at_a_time"]
+        res["p99_ratio"] = round(p["p99_ms"] / c["p99_ms"], 2) if c["p99_ms"] else None
+        by_rate.append(res)
+    return {"requests": n_requests, "by_rate": by_rate,
+            "identical": all(r.get("identical", False) for r in by_rate)}
 
 
 def run_decompose(args, spec) -> dict:
