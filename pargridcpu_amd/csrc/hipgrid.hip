@@ -15,7 +15,7 @@
 //   deg uint32 [n][L], cnt int32 [n], idx int32 [n] (internal ids),
 //   per-block alive list and removal ring uint32 [G*cap] each (block b of
 //   the fixed grid owns vertices [b*cap, (b+1)*cap)), two deferred chunk
-//   lists uint64 [pcap] (pcap = sum over v of ceil(deg(v) / kDefer)).
+//   lists uint64 [pcap] (pcap = sum over v of ceil(deg(v) / kMinDefer)).
 // The graph stays resident between calls; hg_decompose touches no host
 // memory except a small control word per batch of levels.
 
@@ -245,7 +245,7 @@ int create_impl(const hg_layer_csr* layers, int32_t L, int64_t n, int32_t offset
     // the prefix sums run over n + 1 items: the extra last item is 0
     HIP_TRY(hipMemsetAsync(tot_s + n, 0, sizeof(uint64_t), g->stream));
     HIP_TRY(hipMemsetAsync(chunk_cnt + n, 0, sizeof(uint64_t), g->stream));
-    hipLaunchKernelGGL((k_prep_rank<uint64_t(kChunk), uint64_t(kDefer), kPullHeavy>), dim3(std::max(grid_n, 1u)),
+    hipLaunchKernelGGL((k_prep_rank<uint64_t(kChunk), uint64_t(kMinDefer), kPullHeavy>), dim3(std::max(grid_n, 1u)),
                        dim3(kBlock), 0, g->stream, tot_s, perm, nzc, g->n, g->inv, g->z, chunk_cnt, prep);
     HIP_TRY(hipGetLastError());
     {
@@ -287,7 +287,7 @@ int create_impl(const hg_layer_csr* layers, int32_t L, int64_t n, int32_t offset
     auto acct = [&](size_t b) { bytes += int64_t(b); };
     acct(8 * n1 + 4 * size_t(n) + size_t(n));  // off, inv, z
     HIP_TRY(dalloc(&g->adj, g->D)); acct(4 * g->D);
-    HIP_TRY(dalloc(&g->adjz, g->D)); acct(g->D);
+    HIP_TRY(dalloc(&g->adjz, g->D + 16)); acct(g->D);  // the pull reads whole 16-byte words
     HIP_TRY(dalloc(&g->deg0, nL)); acct(4 * nL);
     const size_t aw = (size_t(n) + 31) / 32;
     HIP_TRY(dalloc(&g->alive, aw * L)); acct(4 * aw * L);
@@ -366,7 +366,9 @@ int create_impl(const hg_layer_csr* layers, int32_t L, int64_t n, int32_t offset
         in.lmagic = g->L == 1 ? 0xFFFFFFFFu : uint32_t((1ull << 32) / g->L);  // div_L needs magic < 2^32
         in.flags = env_int("HG_AGG", 1) == 2 ? 0u : kFlagAggregate;  // HG_AGG=2 disables combining
         in.heavy = uint32_t(env_int("HG_HEAVY", int(kHeavy)));
-        in.pad = 0;
+        // deferred chunk size; the lists are sized for chunks of kMinDefer
+        in.defer = std::min<uint32_t>(std::max<uint32_t>(uint32_t(env_int("HG_DEFER", int(kDefer))), kMinDefer),
+                                      1u << 23);
     }
     return HG_OK;
 }

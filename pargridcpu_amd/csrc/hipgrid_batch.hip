@@ -132,6 +132,7 @@ struct Prep {
     std::vector<uint32_t> perm, inv;
     std::vector<uint8_t> zc, zr;           // z in caller's / internal ids
     std::vector<int64_t> off64;            // merged offsets, internal ids
+    std::vector<uint64_t> ent_z;           // [z]: entries of the vertices with z layers, then suffix sums
     uint32_t H = 0, max_nz = 0;
     uint64_t D = 0, maxdeg = 0;
 };
@@ -178,6 +179,11 @@ int prepare(const hg_batch_graph& gr, int32_t gi, Prep& P) {
         maxtot = std::max(maxtot, t);
         P.max_nz = std::max(P.max_nz, zz);
     }
+    // entries of the vertices that take part in instance lmd (z >= lmd):
+    // ent_z[lmd] after the suffix sums
+    P.ent_z.assign(size_t(L) + 2, 0);
+    for (int64_t v = 0; v < n; ++v) P.ent_z[P.zc[size_t(v)]] += tot[size_t(v)];
+    for (int z = L - 1; z >= 0; --z) P.ent_z[size_t(z)] += P.ent_z[size_t(z) + 1];
     // internal id = rank by total degree, descending, stable in the caller's id
     P.perm.assign(size_t(n), 0);
     P.inv.assign(size_t(n), 0);
@@ -296,8 +302,10 @@ int create_impl(const hg_batch_graph* gs, int32_t count, void* stream, hg_batch*
             SH += uint64_t(b->L[g]) * b->kcap[g];
             b->item_of[g][lmd - 1] = int32_t(b->items.size());
             b->items.push_back(it);
-            // one pass over the adjacency per level; smaller lmd peel longer
-            b->cost.push_back(prep[g].D * uint64_t(b->L[g] - lmd + 1) + uint64_t(b->n[g]) * b->L[g]);
+            // a chain visits the entries of its vertices (those with z >= lmd)
+            // about once over all its levels, whatever lmd: measured chain
+            // times of one graph differ by under 10 % across lmd
+            b->cost.push_back(prep[g].ent_z[lmd] + uint64_t(b->n[g]));
         }
     }
     b->hist_words = SH;

@@ -68,8 +68,9 @@ constexpr int kWave = 64;
 constexpr int kBlock = 512;  // threads per block of the level kernels (8 wavefronts)
 constexpr int kWavesPerBlock = kBlock / kWave;
 constexpr int kChunk = 64;         // static chunk list granularity (densest pass)
-constexpr uint32_t kHeavy = 2048;  // default: vertices with more entries are deferred
-constexpr uint32_t kDefer = 4096;  // entries per deferred chunk
+constexpr uint32_t kHeavy = 1024;  // default: vertices with more entries are deferred
+constexpr uint32_t kDefer = 1024;     // default entries per deferred chunk
+constexpr uint32_t kMinDefer = 256;   // smallest chunk size the lists are sized for
 constexpr uint32_t kRun = 64;      // consecutive vertex ids per ownership run
 
 enum Phase : int32_t { kPhaseScan = 0, kPhaseCont = 1, kPhaseDone = 2 };
@@ -93,7 +94,7 @@ struct Inst {
     uint32_t lmagic;  // min(floor(2^32 / L), 2^32 - 1)
     uint32_t flags;   // kFlagAggregate: combine decrements in LDS
     uint32_t heavy;   // cascade removals with more entries are deferred to the next step
-    uint32_t pad;
+    uint32_t defer;   // entries per deferred chunk (>= kMinDefer)
 };
 
 __device__ __forceinline__ uint32_t* rec_of(const Inst& in, uint32_t v) { return in.deg + uint64_t(v) * in.L; }
@@ -206,7 +207,7 @@ __device__ __forceinline__ void defer_push(const Inst& in, uint64_t* list, unsig
     if (defer) {
         beg = uint64_t(in.off[v]);
         dt = uint64_t(in.off[v + 1]) - beg;
-        nch = uint32_t((dt + kDefer - 1) / kDefer);
+        nch = uint32_t((dt + in.defer - 1) / in.defer);
     }
     if (__ballot(nch != 0) == 0) return;
     uint32_t total;
@@ -219,8 +220,8 @@ __device__ __forceinline__ void defer_push(const Inst& in, uint64_t* list, unsig
         return;
     }
     for (uint32_t c = 0; c < nch; ++c) {
-        const uint64_t rem = dt - uint64_t(c) * kDefer;
-        list[base + ex + c] = make_item(beg + uint64_t(c) * kDefer, rem < kDefer ? uint32_t(rem) : kDefer);
+        const uint64_t rem = dt - uint64_t(c) * in.defer;
+        list[base + ex + c] = make_item(beg + uint64_t(c) * in.defer, rem < in.defer ? uint32_t(rem) : in.defer);
     }
 }
 
@@ -234,7 +235,7 @@ __device__ __forceinline__ void block_defer_push(const Inst& in, uint64_t* list,
     if (defer) {
         beg = uint64_t(in.off[v]);
         dt = uint64_t(in.off[v + 1]) - beg;
-        nch = uint32_t((dt + kDefer - 1) / kDefer);
+        nch = uint32_t((dt + in.defer - 1) / in.defer);
     }
     uint32_t total;
     const uint32_t ex = block_excl_scan(nch, &total, s_tmp);
@@ -247,8 +248,8 @@ __device__ __forceinline__ void block_defer_push(const Inst& in, uint64_t* list,
         return;
     }
     for (uint32_t c = 0; c < nch; ++c) {
-        const uint64_t rem = dt - uint64_t(c) * kDefer;
-        list[base + ex + c] = make_item(beg + uint64_t(c) * kDefer, rem < kDefer ? uint32_t(rem) : kDefer);
+        const uint64_t rem = dt - uint64_t(c) * in.defer;
+        list[base + ex + c] = make_item(beg + uint64_t(c) * in.defer, rem < in.defer ? uint32_t(rem) : in.defer);
     }
 }
 
@@ -496,19 +497,53 @@ constexpr uint32_t kPullMaxL = 16;
 constexpr uint64_t kPullHeavy = 1024;  // vertices with more entries are pulled by a wavefront
 constexpr uint32_t kStageWords = 32768;  // level-0 row staging: instances * layers * kBlock <= this
 
-// c[j] += #{entries w of seg : z(w) > j}, j < kPullMaxL (segz = the
-// entries' target z, streamed sequentially).
-__device__ __forceinline__ void pull_count(const uint8_t* segz, uint32_t beg, uint32_t end, uint32_t step,
-                                           uint32_t* c) {
-    for (uint32_t t = beg; t < end; t += step) {
-        const uint32_t zw = segz[t];
+// Level-0 counts: c[j] += #{entries of a segment whose target z > j},
+// j < kPullMaxL, byte-parallel over 16-byte words of the z stream (z values
+// <= kPullMaxL, so byte + 127 - j >= 128 iff byte > j and no byte carries
+// into the next): for each j, one add, one mask and a popcount per 4 bytes.
+// sel holds 0x80 in every byte that belongs to the segment.
+__device__ __forceinline__ void count_word(uint32_t w, uint32_t sel, uint32_t* c) {
 #pragma unroll
-        for (uint32_t j = 0; j < kPullMaxL; ++j) c[j] += zw > j;
+    for (uint32_t j = 0; j < kPullMaxL; ++j) c[j] += uint32_t(__popc((w + (0x7F7F7F7Fu - j * 0x01010101u)) & sel));
+}
+
+// 0x80 in each byte t of the 4-byte word starting at byte b0 with beg <= t < end.
+__device__ __forceinline__ uint32_t byte_sel(uint64_t b0, uint64_t beg, uint64_t end) {
+    uint32_t sel = 0;
+#pragma unroll
+    for (uint32_t b = 0; b < 4; ++b)
+        if (b0 + b >= beg && b0 + b < end) sel |= 0x80u << (8 * b);
+    return sel;
+}
+
+// c[j] += #{t in [beg, end) : z[t] > j}, reading the 16-byte words of z
+// that cover the range (z 16-byte aligned and readable up to the word that
+// holds end - 1), word q0 + first, q0 + first + step, ...
+__device__ __forceinline__ void pull_count_vec(const uint8_t* z, uint64_t beg, uint64_t end, uint32_t first,
+                                               uint32_t step, uint32_t* c) {
+    if (beg >= end) return;
+    const uint4* zp = reinterpret_cast<const uint4*>(z);
+    const uint64_t q1 = (end - 1) >> 4;
+    for (uint64_t q = (beg >> 4) + first; q <= q1; q += step) {
+        const uint4 x = zp[q];
+        const uint64_t b0 = q << 4;
+        if (b0 >= beg && b0 + 16 <= end) {  // whole word inside the segment
+            count_word(x.x, 0x80808080u, c);
+            count_word(x.y, 0x80808080u, c);
+            count_word(x.z, 0x80808080u, c);
+            count_word(x.w, 0x80808080u, c);
+        } else {
+            count_word(x.x, byte_sel(b0, beg, end), c);
+            count_word(x.y, byte_sel(b0 + 4, beg, end), c);
+            count_word(x.z, byte_sel(b0 + 8, beg, end), c);
+            count_word(x.w, byte_sel(b0 + 12, beg, end), c);
+        }
     }
 }
 
 // Hubs (ids < H, the heaviest vertices after the degree sort): one
-// wavefront per vertex, lanes strided over each layer segment.
+// wavefront per vertex, lanes strided over the 16-byte words of each layer
+// segment.
 __global__ void __launch_bounds__(kBlock) k_pull_heavy(const Inst* insts, uint32_t I, const uint32_t* deg0,
                                                        const uint8_t* z, uint32_t H, uint32_t L) {
     const uint32_t lane = lane_id();
@@ -523,7 +558,7 @@ __global__ void __launch_bounds__(kBlock) k_pull_heavy(const Inst* insts, uint32
             uint32_t c[kPullMaxL];
 #pragma unroll
             for (uint32_t j = 0; j < kPullMaxL; ++j) c[j] = 0;
-            pull_count(in0.adjz + pos, lane, d, kWave, c);
+            pull_count_vec(in0.adjz, pos, pos + d, lane, kWave, c);
             pos += d;
 #pragma unroll
             for (uint32_t j = 0; j < kPullMaxL; ++j)
@@ -584,7 +619,7 @@ __global__ void __launch_bounds__(kBlock) k_init(const Inst* insts, InstState* s
                     uint32_t c[kPullMaxL];
 #pragma unroll
                     for (uint32_t j = 0; j < kPullMaxL; ++j) c[j] = 0;
-                    pull_count(in0.adjz + pos, 0, d, 1, c);
+                    pull_count_vec(in0.adjz, pos, pos + d, 0, 1, c);
                     pos += d;
 #pragma unroll
                     for (uint32_t j = 0; j < kPullMaxL; ++j) {
@@ -744,6 +779,9 @@ constexpr uint32_t kAggEmpty = 0xFFFFFFFFu;  // entries are < n*L <= 2^32 - 1
 constexpr uint32_t kAggProbes = 8;
 
 struct CascShared {
+    int32_t phase[64];            // per instance: this step's phase, level and chunk count
+    int32_t level[64];
+    unsigned long long chunks[64];
     uint32_t pre[kBlock];
     uint64_t start[kBlock];
     uint32_t key[kAggSlots];
@@ -954,18 +992,26 @@ __global__ void __launch_bounds__(kBlock) k_cascade(const Inst* insts, InstState
                                                     uint32_t s) {
     __shared__ CascShared sh;
     const uint32_t G = gridDim.x, b = blockIdx.x;
+    const uint32_t par = s & 1u, npar = par ^ 1u;
     for (uint32_t t = threadIdx.x; t < kAggSlots; t += kBlock) {
         sh.key[t] = kAggEmpty;
         sh.cnt[t] = 0;
     }
     if (threadIdx.x == 0) sh.used = 0;
+    // the step's slots of every instance in one parallel load (written by
+    // the preceding level kernel, read-only here)
+    if (threadIdx.x < I) {
+        const InstState* st = &states[threadIdx.x];
+        sh.phase[threadIdx.x] = st->dphase[par];
+        sh.level[threadIdx.x] = st->dk[par];
+        sh.chunks[threadIdx.x] = st->pdefer[par][0] + st->pfront[par][0];  // one of them is zero
+    }
+    __syncthreads();
     uint64_t seq = 0;  // chunks of the instances before i
     for (uint32_t i = 0; i < I; ++i) {
         InstState* st = &states[i];
-        const uint32_t par = s & 1u, npar = par ^ 1u;
-        const int32_t phase = st->dphase[par];
-        if (phase == kPhaseDone) continue;
-        const uint64_t pc = st->pdefer[par][0] + st->pfront[par][0];  // one of them is zero
+        if (sh.phase[i] == kPhaseDone) continue;
+        const uint64_t pc = sh.chunks[i];
         if (pc == 0) continue;  // empty frontier: the ring starts empty, nothing to do
         // chunk j of instance i is item seq + j of the sequence; it goes to
         // block (seq + j) mod G, so this block's first chunk is j = first
@@ -973,7 +1019,7 @@ __global__ void __launch_bounds__(kBlock) k_cascade(const Inst* insts, InstState
         seq += pc;
         if (first >= pc) continue;
         const Inst& in = insts[i];
-        const int32_t k = st->dk[par];
+        const int32_t k = sh.level[i];
         uint32_t* ring = in.ring + uint64_t(b) * in.cap;
         if (threadIdx.x == 0) {
             sh.head = 0;

@@ -2,7 +2,7 @@
 
 Bar: bit-exact core indices, cell lengths, k-shells and WdsList results.
 Full-size workloads are checked through size-independent properties in
-test_gpu_properties.py; here the oracle finishes in seconds.
+test_gpu_golden.py (test_full_size_c2_properties); here the oracle finishes in seconds.
 """
 import dataclasses
 import sys
