@@ -532,12 +532,28 @@ __device__ __forceinline__ void pull_count_vec(const uint8_t* z, uint64_t beg, u
             count_word(x.y, 0x80808080u, c);
             count_word(x.z, 0x80808080u, c);
             count_word(x.w, 0x80808080u, c);
-        } else {
-            count_word(x.x, byte_sel(b0, beg, end), c);
-            count_word(x.y, byte_sel(b0 + 4, beg, end), c);
-            count_word(x.z, byte_sel(b0 + 8, beg, end), c);
-            count_word(x.w, byte_sel(b0 + 12, beg, end), c);
+        } else {  // a word at an end of the segment: only its selected dwords
+            const uint32_t w[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+            for (uint32_t k = 0; k < 4; ++k) {
+                const uint32_t sel = byte_sel(b0 + 4 * k, beg, end);
+                if (sel) count_word(w[k], sel, c);
+            }
         }
+    }
+}
+
+// The same count for one thread's segment: short segments (the common
+// case, a few entries per layer) byte by byte, long ones by words.
+__device__ __forceinline__ void pull_count_seg(const uint8_t* z, uint64_t beg, uint64_t end, uint32_t* c) {
+    if (end - beg < 16) {
+        for (uint64_t t = beg; t < end; ++t) {
+            const uint32_t zw = z[t];
+#pragma unroll
+            for (uint32_t j = 0; j < kPullMaxL; ++j) c[j] += zw > j;
+        }
+    } else {
+        pull_count_vec(z, beg, end, 0, 1, c);
     }
 }
 
@@ -619,7 +635,7 @@ __global__ void __launch_bounds__(kBlock) k_init(const Inst* insts, InstState* s
                     uint32_t c[kPullMaxL];
 #pragma unroll
                     for (uint32_t j = 0; j < kPullMaxL; ++j) c[j] = 0;
-                    pull_count_vec(in0.adjz, pos, pos + d, 0, 1, c);
+                    pull_count_seg(in0.adjz, pos, pos + d, c);
                     pos += d;
 #pragma unroll
                     for (uint32_t j = 0; j < kPullMaxL; ++j) {
