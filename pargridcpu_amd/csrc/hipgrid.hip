@@ -392,14 +392,25 @@ const char* err_name(int32_t e) {
 void launch_step(hg_graph* g, uint32_t s) {
     const uint32_t L = g->L;
     if (s == 0) {
+        // level-0 counters per thread: 8 when the graph has at most 8 layers
+        // (fewer registers, so three blocks fit a CU), else 16
+        const bool narrow = L <= 8;
         if (g->pull && g->H > 0) {
             const uint32_t gh = uint32_t(std::min<uint64_t>((uint64_t(g->H) * kWave + kBlock - 1) / kBlock,
                                                             uint64_t(g->cus) * 8));
-            hipLaunchKernelGGL(k_pull_heavy, dim3(gh), dim3(kBlock), 0, g->stream, g->d_insts, L, g->deg0, g->z, g->H,
-                               g->L);
+            if (narrow)
+                hipLaunchKernelGGL(k_pull_heavy<8>, dim3(gh), dim3(kBlock), 0, g->stream, g->d_insts, L, g->deg0, g->z,
+                                   g->H, g->L);
+            else
+                hipLaunchKernelGGL(k_pull_heavy<16>, dim3(gh), dim3(kBlock), 0, g->stream, g->d_insts, L, g->deg0,
+                                   g->z, g->H, g->L);
         }
-        hipLaunchKernelGGL(k_init, dim3(g->G), dim3(kBlock), 0, g->stream, g->d_insts, g->d_states, g->d_ctl, L,
-                           g->deg0, g->z, g->n, g->L, g->cap, g->H, g->pull);
+        if (narrow)
+            hipLaunchKernelGGL(k_init<8>, dim3(g->G), dim3(kBlock), 0, g->stream, g->d_insts, g->d_states, g->d_ctl, L,
+                               g->deg0, g->z, g->n, g->L, g->cap, g->H, g->pull);
+        else
+            hipLaunchKernelGGL(k_init<16>, dim3(g->G), dim3(kBlock), 0, g->stream, g->d_insts, g->d_states, g->d_ctl,
+                               L, g->deg0, g->z, g->n, g->L, g->cap, g->H, g->pull);
     }
     else
         hipLaunchKernelGGL(k_scan, dim3(g->G), dim3(kBlock), 0, g->stream, g->d_insts, g->d_states, g->d_ctl, L, s);

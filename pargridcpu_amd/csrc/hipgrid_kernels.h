@@ -494,7 +494,9 @@ __device__ __forceinline__ void scan_emit(const Inst& in, InstState* st, Control
 // level-0 scan then removes the survivors with fewer than lmd non-empty
 // layers; only those are pushed (their neighbours counted them).
 constexpr uint32_t kPullMaxL = 16;
-constexpr uint64_t kPullHeavy = 1024;  // vertices with more entries are pulled by a wavefront
+// Vertices with more entries are pulled by a wavefront (k_pull_heavy); the
+// others by a thread, and their per-layer counts (<= kPullHeavy) fit a byte.
+constexpr uint64_t kPullHeavy = 255;
 constexpr uint32_t kStageWords = 32768;  // level-0 row staging: instances * layers * kBlock <= this
 
 // Level-0 counts: c[j] += #{entries of a segment whose target z > j},
@@ -502,9 +504,11 @@ constexpr uint32_t kStageWords = 32768;  // level-0 row staging: instances * lay
 // <= kPullMaxL, so byte + 127 - j >= 128 iff byte > j and no byte carries
 // into the next): for each j, one add, one mask and a popcount per 4 bytes.
 // sel holds 0x80 in every byte that belongs to the segment.
+// NJ: counters kept (instances of the graph, rounded up to 8 or 16).
+template <uint32_t NJ>
 __device__ __forceinline__ void count_word(uint32_t w, uint32_t sel, uint32_t* c) {
 #pragma unroll
-    for (uint32_t j = 0; j < kPullMaxL; ++j) c[j] += uint32_t(__popc((w + (0x7F7F7F7Fu - j * 0x01010101u)) & sel));
+    for (uint32_t j = 0; j < NJ; ++j) c[j] += uint32_t(__popc((w + (0x7F7F7F7Fu - j * 0x01010101u)) & sel));
 }
 
 // 0x80 in each byte t of the 4-byte word starting at byte b0 with beg <= t < end.
@@ -519,6 +523,7 @@ __device__ __forceinline__ uint32_t byte_sel(uint64_t b0, uint64_t beg, uint64_t
 // c[j] += #{t in [beg, end) : z[t] > j}, reading the 16-byte words of z
 // that cover the range (z 16-byte aligned and readable up to the word that
 // holds end - 1), word q0 + first, q0 + first + step, ...
+template <uint32_t NJ>
 __device__ __forceinline__ void pull_count_vec(const uint8_t* z, uint64_t beg, uint64_t end, uint32_t first,
                                                uint32_t step, uint32_t* c) {
     if (beg >= end) return;
@@ -528,16 +533,16 @@ __device__ __forceinline__ void pull_count_vec(const uint8_t* z, uint64_t beg, u
         const uint4 x = zp[q];
         const uint64_t b0 = q << 4;
         if (b0 >= beg && b0 + 16 <= end) {  // whole word inside the segment
-            count_word(x.x, 0x80808080u, c);
-            count_word(x.y, 0x80808080u, c);
-            count_word(x.z, 0x80808080u, c);
-            count_word(x.w, 0x80808080u, c);
+            count_word<NJ>(x.x, 0x80808080u, c);
+            count_word<NJ>(x.y, 0x80808080u, c);
+            count_word<NJ>(x.z, 0x80808080u, c);
+            count_word<NJ>(x.w, 0x80808080u, c);
         } else {  // a word at an end of the segment: only its selected dwords
             const uint32_t w[4] = {x.x, x.y, x.z, x.w};
 #pragma unroll
             for (uint32_t k = 0; k < 4; ++k) {
                 const uint32_t sel = byte_sel(b0 + 4 * k, beg, end);
-                if (sel) count_word(w[k], sel, c);
+                if (sel) count_word<NJ>(w[k], sel, c);
             }
         }
     }
@@ -545,21 +550,23 @@ __device__ __forceinline__ void pull_count_vec(const uint8_t* z, uint64_t beg, u
 
 // The same count for one thread's segment: short segments (the common
 // case, a few entries per layer) byte by byte, long ones by words.
+template <uint32_t NJ>
 __device__ __forceinline__ void pull_count_seg(const uint8_t* z, uint64_t beg, uint64_t end, uint32_t* c) {
     if (end - beg < 16) {
         for (uint64_t t = beg; t < end; ++t) {
             const uint32_t zw = z[t];
 #pragma unroll
-            for (uint32_t j = 0; j < kPullMaxL; ++j) c[j] += zw > j;
+            for (uint32_t j = 0; j < NJ; ++j) c[j] += zw > j;
         }
     } else {
-        pull_count_vec(z, beg, end, 0, 1, c);
+        pull_count_vec<NJ>(z, beg, end, 0, 1, c);
     }
 }
 
 // Hubs (ids < H, the heaviest vertices after the degree sort): one
 // wavefront per vertex, lanes strided over the 16-byte words of each layer
 // segment.
+template <uint32_t NJ>
 __global__ void __launch_bounds__(kBlock) k_pull_heavy(const Inst* insts, uint32_t I, const uint32_t* deg0,
                                                        const uint8_t* z, uint32_t H, uint32_t L) {
     const uint32_t lane = lane_id();
@@ -571,13 +578,13 @@ __global__ void __launch_bounds__(kBlock) k_pull_heavy(const Inst* insts, uint32
         uint64_t pos = uint64_t(in0.off[v]);
         for (uint32_t l = 0; l < L; ++l) {
             const uint32_t d = deg0[uint64_t(v) * L + l];
-            uint32_t c[kPullMaxL];
+            uint32_t c[NJ];
 #pragma unroll
-            for (uint32_t j = 0; j < kPullMaxL; ++j) c[j] = 0;
-            pull_count_vec(in0.adjz, pos, pos + d, lane, kWave, c);
+            for (uint32_t j = 0; j < NJ; ++j) c[j] = 0;
+            pull_count_vec<NJ>(in0.adjz, pos, pos + d, lane, kWave, c);
             pos += d;
 #pragma unroll
-            for (uint32_t j = 0; j < kPullMaxL; ++j)
+            for (uint32_t j = 0; j < NJ; ++j)
                 for (int o = kWave / 2; o > 0; o >>= 1) c[j] += __shfl_xor(c[j], o, kWave);
             if (lane == 0)
                 for (uint32_t i = 0; i < I && i < zv; ++i) insts[i].deg[uint64_t(v) * L + l] = c[i];
@@ -589,16 +596,22 @@ __global__ void __launch_bounds__(kBlock) k_pull_heavy(const Inst* insts, uint32
 // L > kPullMaxL), cnt, index, liveness bit and alive list; removed vertices
 // that neighbours still count go to the frontier list.  H: vertices below
 // it were pulled by k_pull_heavy.
-__global__ void __launch_bounds__(kBlock) k_init(const Inst* insts, InstState* states, Control* ctl, uint32_t I,
+// Three blocks per CU (the grid of every level kernel, all resident at
+// once): six wavefronts per SIMD, so at most 80 VGPRs.
+template <uint32_t NJ>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6)))
+k_init(const Inst* insts, InstState* states, Control* ctl, uint32_t I,
                                                  const uint32_t* deg0, const uint8_t* z, uint32_t n, uint32_t L,
                                                  uint32_t cap, uint32_t H, uint32_t pull) {
     __shared__ ScanShared sh[64];
     __shared__ StepShared ss;
-    // Pulled rows of light vertices are staged here (16 bits suffice: a light
+    // Pulled rows of light vertices are staged here (a byte suffices: a light
     // vertex has at most kPullHeavy entries) and written out as whole runs of
-    // 64 rows, so every line of deg is written once, in full.
-    __shared__ uint16_t s_stage[kStageWords];   // [instance][candidate][layer]
-    __shared__ uint8_t s_c1[kPullMaxL * kBlock];  // layers with a pulled count >= 1
+    // 64 rows, so every line of deg is written once, in full.  32 KB of
+    // staging leaves room for three blocks per CU, the grid of every level
+    // kernel (all of them resident at once).
+    __shared__ uint8_t s_stage[kStageWords];   // [instance][candidate][layer]
+    __shared__ uint8_t s_c1[NJ * kBlock];  // layers with a pulled count >= 1
     const bool stage = pull && I * L * uint32_t(kBlock) <= kStageWords;
     for (uint32_t i = threadIdx.x; i < I; i += blockDim.x) {
         sh[i].keep = 0;
@@ -627,27 +640,27 @@ __global__ void __launch_bounds__(kBlock) k_init(const Inst* insts, InstState* s
                 }
             } else if (v >= H) {
                 uint64_t pos = uint64_t(in0.off[v]);
-                uint32_t c1v[kPullMaxL];
+                uint32_t c1v[NJ];
 #pragma unroll
-                for (uint32_t j = 0; j < kPullMaxL; ++j) c1v[j] = 0;
+                for (uint32_t j = 0; j < NJ; ++j) c1v[j] = 0;
                 for (uint32_t l = 0; l < L; ++l) {
                     const uint32_t d = deg0[uint64_t(v) * L + l];
-                    uint32_t c[kPullMaxL];
+                    uint32_t c[NJ];
 #pragma unroll
-                    for (uint32_t j = 0; j < kPullMaxL; ++j) c[j] = 0;
-                    pull_count_seg(in0.adjz, pos, pos + d, c);
+                    for (uint32_t j = 0; j < NJ; ++j) c[j] = 0;
+                    pull_count_seg<NJ>(in0.adjz, pos, pos + d, c);
                     pos += d;
 #pragma unroll
-                    for (uint32_t j = 0; j < kPullMaxL; ++j) {
+                    for (uint32_t j = 0; j < NJ; ++j) {
                         if (j >= I) continue;
                         c1v[j] += c[j] >= 1u;
-                        if (stage) s_stage[(j * kBlock + threadIdx.x) * L + l] = uint16_t(j < zv ? c[j] : 0u);
+                        if (stage) s_stage[(j * kBlock + threadIdx.x) * L + l] = uint8_t(j < zv ? c[j] : 0u);
                         else if (j < zv) insts[j].deg[uint64_t(v) * L + l] = c[j];
                     }
                 }
                 if (stage) {
 #pragma unroll
-                    for (uint32_t j = 0; j < kPullMaxL; ++j)
+                    for (uint32_t j = 0; j < NJ; ++j)
                         if (j < I) s_c1[j * kBlock + threadIdx.x] = uint8_t(j < zv ? c1v[j] : 0u);
                 }
             }
