@@ -182,7 +182,8 @@ typedef struct hg_batch_graph {
 typedef struct hg_batch_params {
     int32_t block;         /* threads per workgroup: 256, 512 or 1024 */
     int32_t unroll;        /* entries per thread in flight per cascade round: 2 or 4 */
-    int32_t blocks_per_cu; /* persistent workgroups per compute unit: 1, 2, 4 or 8 */
+    int32_t blocks_per_cu; /* persistent workgroups per compute unit: 1, 2, 4 or 8 (capped at what
+                              one CU holds resident for this block size) */
     int32_t order;         /* 0: items by estimated cost, largest first; 1: graph order */
     int32_t use_graph;     /* 1: replay a captured hipGraph; 0: plain launches */
     int32_t reserved;

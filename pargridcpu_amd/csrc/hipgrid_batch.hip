@@ -475,9 +475,32 @@ void launch_peel_t(hg_batch* b, uint32_t grid) {
                        b->degpool, b->cntpool, b->idxpool, b->listpool, b->qpool, b->bitpool, b->out, b->d_kmax);
 }
 
+template <int B, int U>
+int resident_t() {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(&k_batch_peel<B, U>), B, 0) !=
+        hipSuccess)
+        return 1;
+    return std::max(nb, 1);
+}
+
+// Workgroups of this launch shape that one CU holds at once (registers and
+// LDS): more per CU would only queue behind the resident ones.
+int resident_per_cu(int block, int unroll) {
+    switch (block * 10 + unroll) {
+        case 2562: return resident_t<256, 2>();
+        case 2564: return resident_t<256, 4>();
+        case 5122: return resident_t<512, 2>();
+        case 5124: return resident_t<512, 4>();
+        case 10242: return resident_t<1024, 2>();
+        default: return resident_t<1024, 4>();
+    }
+}
+
 void launch_peel(hg_batch* b) {
     hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, b->stream, b->d_head);
-    const uint32_t want = uint32_t(b->cus) * uint32_t(b->params.blocks_per_cu);
+    const int bpc = std::min(b->params.blocks_per_cu, resident_per_cu(b->params.block, b->params.unroll));
+    const uint32_t want = uint32_t(b->cus) * uint32_t(std::max(bpc, 1));
     const uint32_t grid = std::max<uint32_t>(1u, std::min<uint32_t>(want, uint32_t(b->items.size())));
     switch (b->params.block * 10 + b->params.unroll) {
         case 2562: launch_peel_t<256, 2>(b, grid); break;
@@ -716,6 +739,7 @@ int hg_batch_tune(hg_batch* b, int32_t reps, hg_batch_params* best, double* best
             for (int u : unrolls)
                 for (int bpc : bpcs) {
                     if (blk * bpc > 2048) continue;  // beyond one CU's resident threads
+                    if (bpc > resident_per_cu(blk, u)) continue;  // would not all be resident
                     hg_batch_params p = default_params();
                     p.block = blk;
                     p.unroll = u;
