@@ -62,6 +62,9 @@ struct hg_batch {
     int32_t* out = nullptr;
     BGraph* d_graphs = nullptr;
     BItem* d_items = nullptr;  // in queue order
+    BItem* d_items_id = nullptr;  // by item id
+    PullTile* d_tiles = nullptr;  // level-0 pull work of the pulled graphs
+    uint32_t ntiles = 0, pull_nj = 0;
     uint32_t* d_head = nullptr;
     int32_t* d_kmax = nullptr;  // by item id
     uint32_t* degpool = nullptr;
@@ -115,9 +118,10 @@ void release(hg_batch* b) {
     if (!b) return;
     (void)hipSetDevice(b->device);
     drop_graph(b);
-    void* ptrs[] = {b->off, b->adj, b->adjz, b->z, b->deg0, b->perm, b->out, b->d_graphs, b->d_items, b->d_head,
-                    b->d_kmax, b->degpool, b->cntpool, b->idxpool, b->listpool, b->qpool, b->bitpool, b->histpool,
-                    b->d_sel};
+    void* ptrs[] = {b->off,     b->adj,     b->adjz,     b->z,        b->deg0,    b->perm,
+                    b->out,     b->d_graphs, b->d_items, b->d_items_id, b->d_tiles, b->d_head,
+                    b->d_kmax,  b->degpool, b->cntpool,  b->idxpool,  b->listpool, b->qpool,
+                    b->bitpool, b->histpool, b->d_sel};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (auto& e : b->ev)
@@ -133,7 +137,7 @@ struct Prep {
     std::vector<uint8_t> zc, zr;           // z in caller's / internal ids
     std::vector<int64_t> off64;            // merged offsets, internal ids
     std::vector<uint64_t> ent_z;           // [z]: entries of the vertices with z layers, then suffix sums
-    uint32_t H = 0, max_nz = 0;
+    uint32_t H = 0, Hp = 0, max_nz = 0;
     uint64_t D = 0, maxdeg = 0;
 };
 
@@ -204,6 +208,7 @@ int prepare(const hg_batch_graph& gr, int32_t gi, Prep& P) {
         P.off64[size_t(r) + 1] = P.off64[size_t(r)] + int64_t(tot[v]);
         P.zr[size_t(r)] = P.zc[v];
         if (tot[v] > kWaveVertex) P.H = uint32_t(r) + 1;
+        if (tot[v] > hg::kPullHeavy) P.Hp = uint32_t(r) + 1;
     }
     P.D = uint64_t(P.off64[size_t(n)]);
     if (P.D >= (1ull << 32)) return fail(HG_ERR_RANGE, tag + "directed entries must be < 2^32");
@@ -284,6 +289,10 @@ int create_impl(const hg_batch_graph* gs, int32_t count, void* stream, hg_batch*
         bg.L = b->L[g];
         bg.lmagic = bg.L == 1 ? 0xFFFFFFFFu : uint32_t((1ull << 32) / bg.L);
         bg.H = prep[g].H;
+        bg.Hp = prep[g].Hp;
+        bg.item0 = uint32_t(b->items.size());
+        bg.nit = std::min(b->L[g], b->max_nz[g]);
+        bg.pulled = bg.nit >= 1 && bg.nit <= hg::kPullMaxL;
         b->item_of[g].assign(b->L[g], -1);
         for (uint32_t lmd = 1; lmd <= b->L[g] && lmd <= b->max_nz[g]; ++lmd) {
             BItem it{};
@@ -310,18 +319,35 @@ int create_impl(const hg_batch_graph* gs, int32_t count, void* stream, hg_batch*
     }
     b->hist_words = SH;
     const size_t I = std::max<size_t>(b->items.size(), 1);
+    // Level-0 pull work: a wavefront per vertex of the heavy head, a thread
+    // per vertex for the rest, in tiles of one graph each.
+    std::vector<PullTile> tiles;
+    uint32_t max_pulled = 0;
+    for (size_t g = 0; g < G; ++g) {
+        const BGraph& bg = b->graphs[g];
+        if (!bg.pulled) continue;
+        max_pulled = std::max(max_pulled, bg.nit);
+        for (uint32_t v = 0; v < bg.Hp; v += kPullHeavyTile)
+            tiles.push_back(PullTile{uint32_t(g), v, std::min(kPullHeavyTile, bg.Hp - v), 1u});
+        for (uint32_t v = bg.Hp; v < bg.n; v += kPullLightTile)
+            tiles.push_back(PullTile{uint32_t(g), v, std::min(kPullLightTile, bg.n - v), 0u});
+    }
+    b->ntiles = uint32_t(tiles.size());
+    b->pull_nj = max_pulled <= 8 ? 8u : 16u;
 
     int64_t bytes = 0;
     auto acct = [&](uint64_t x) { bytes += int64_t(x); };
     HIP_TRY(dalloc(&b->off, O)); acct(4 * O);
     HIP_TRY(dalloc(&b->adj, E)); acct(4 * E);
-    HIP_TRY(dalloc(&b->adjz, E)); acct(E);
+    HIP_TRY(dalloc(&b->adjz, E + 16)); acct(E + 16);  // 16-byte reads past the last entry
     HIP_TRY(dalloc(&b->z, V)); acct(V);
     HIP_TRY(dalloc(&b->deg0, DB)); acct(4 * DB);
     HIP_TRY(dalloc(&b->perm, V)); acct(4 * V);
     HIP_TRY(dalloc(&b->out, OUT)); acct(4 * OUT);
     HIP_TRY(dalloc(&b->d_graphs, G));
     HIP_TRY(dalloc(&b->d_items, I));
+    HIP_TRY(dalloc(&b->d_items_id, I));
+    HIP_TRY(dalloc(&b->d_tiles, std::max<size_t>(tiles.size(), 1)));
     HIP_TRY(dalloc(&b->d_head, 1));
     HIP_TRY(dalloc(&b->d_kmax, I));
     HIP_TRY(dalloc(&b->degpool, SDEG)); acct(4 * SDEG);
@@ -404,6 +430,9 @@ int create_impl(const hg_batch_graph* gs, int32_t count, void* stream, hg_batch*
     if (e == hipSuccess) e = up.h2d(b->perm, h_perm.data(), 4 * V);
     if (e == hipSuccess) e = hipMemsetAsync(b->out, 0, 4 * std::max<uint64_t>(OUT, 1), b->stream);
     if (e == hipSuccess) e = up.h2d(b->d_graphs, b->graphs.data(), sizeof(BGraph) * G);
+    if (e == hipSuccess && !b->items.empty()) e = up.h2d(b->d_items_id, b->items.data(), sizeof(BItem) * b->items.size());
+    if (e == hipSuccess && !tiles.empty()) e = up.h2d(b->d_tiles, tiles.data(), sizeof(PullTile) * tiles.size());
+    if (e == hipSuccess) e = hipMemsetAsync(b->adjz + E, 0, 16, b->stream);
     for (size_t g = 0; g < G && e == hipSuccess; ++g) {
         const uint64_t nL = uint64_t(b->n[g]) * b->L[g];
         const uint32_t grid = uint32_t(std::min<uint64_t>((nL + hg::kBlock - 1) / hg::kBlock, uint64_t(b->cus) * 16));
@@ -498,6 +527,17 @@ int resident_per_cu(int block, int unroll) {
 }
 
 void launch_peel(hg_batch* b) {
+    if (b->ntiles) {
+        const uint32_t grid = std::min<uint32_t>(b->ntiles, uint32_t(b->cus) * 16);
+        if (b->pull_nj == 8)
+            hipLaunchKernelGGL(k_batch_pull<8>, dim3(grid), dim3(kPullBlock), 0, b->stream, b->d_graphs,
+                               b->d_items_id, b->d_tiles, b->ntiles, b->off, b->adjz, b->z, b->deg0, b->degpool,
+                               b->cntpool);
+        else
+            hipLaunchKernelGGL(k_batch_pull<16>, dim3(grid), dim3(kPullBlock), 0, b->stream, b->d_graphs,
+                               b->d_items_id, b->d_tiles, b->ntiles, b->off, b->adjz, b->z, b->deg0, b->degpool,
+                               b->cntpool);
+    }
     hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, b->stream, b->d_head);
     const int bpc = std::min(b->params.blocks_per_cu, resident_per_cu(b->params.block, b->params.unroll));
     const uint32_t want = uint32_t(b->cus) * uint32_t(std::max(bpc, 1));

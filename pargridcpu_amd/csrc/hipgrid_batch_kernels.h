@@ -58,6 +58,9 @@ struct BGraph {
     uint64_t ob;  // first offset of the graph in off (n + 1 entries, local to eb)
     uint64_t db;  // first word of the graph's deg0 rows [n][L]
     uint32_t n, L, lmagic, H;  // H: vertices with more than kWaveVertex entries (ids < H)
+    uint32_t item0, nit;       // the graph's items: ids item0 .. item0 + nit - 1, lmd = 1 .. nit
+    uint32_t pulled;           // 1: k_batch_pull writes level 0 of every item (nit <= kPullMaxL)
+    uint32_t Hp;               // vertices with more than kPullHeavy entries (ids < Hp)
 };
 
 struct BItem {
@@ -107,8 +110,20 @@ __device__ __forceinline__ void sync_global() {
 }
 
 
+// Degree words of the item's heaviest vertices (internal ids < T, T * L <=
+// the table) live in LDS for the whole chain: their decrements are LDS
+// atomics instead of device-scope ones.  Ids are sorted by total degree, so
+// the table holds the rows that take the most decrements (measured on
+// graphs of 90k-180k vertices: 17-32 % of all decrements with 96 KB, most
+// of them for graphs of a few ten thousand vertices).
+template <int B>
+constexpr uint32_t hub_words() {
+    return B >= 1024 ? 24576u : B >= 512 ? 8192u : 4096u;
+}
+
 template <int B>
 struct PeelShared {
+    uint32_t hub[hub_words<B>()];
     uint32_t bits[kLdsWords];
     uint32_t pre[B];
     uint32_t start[B];
@@ -136,7 +151,9 @@ struct ItemView {
     uint32_t* queue;
     uint32_t* bits;        // generic pointer: LDS or global
     bool lds_bits;
+    bool pulled;           // level-0 rows and cnt written by k_batch_pull
     uint32_t n, L, lmd, magic, H;
+    uint32_t T, TL;        // vertices < T have their degree rows in LDS (words < TL = T * L)
 };
 
 __device__ __forceinline__ bool bit_get(const ItemView& v, uint32_t u) {
@@ -165,8 +182,10 @@ __device__ void peel_level0(const ItemView& v, PeelShared<B>* sh) {
     const uint32_t n = v.n, L = v.L, lmd = v.lmd;
     const uint32_t lane = lane_id(), wid = threadIdx.x / kWave;
     constexpr uint32_t W = B / kWave;
+    const uint32_t H = v.pulled ? 0u : v.H;
+    const uint32_t n_pull = v.pulled ? 0u : n;
     // heavy head: a wavefront per vertex
-    for (uint32_t u = wid; u < v.H; u += W) {
+    for (uint32_t u = wid; u < H; u += W) {
         if (v.z[u] < lmd) {
             if (lane == 0) v.cnt[u] = 0;
             continue;
@@ -184,7 +203,7 @@ __device__ void peel_level0(const ItemView& v, PeelShared<B>* sh) {
         if (lane == 0) v.cnt[u] = int32_t(c1);
     }
     // the rest: a thread per vertex
-    for (uint32_t u = v.H + threadIdx.x; u < n; u += B) {
+    for (uint32_t u = H + threadIdx.x; u < n_pull; u += B) {
         uint32_t c1 = 0;
         if (v.z[u] >= lmd) {
             uint32_t pos = v.off[u];
@@ -230,6 +249,8 @@ __device__ void peel_level0(const ItemView& v, PeelShared<B>* sh) {
                 v.queue[qb + __builtin_amdgcn_mbcnt_hi(uint32_t(qm >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(qm), 0u))] = u;
         }
     }
+    // the hub rows move to LDS (rows of vertices with z < lmd are never read)
+    for (uint32_t e = threadIdx.x; e < v.TL; e += B) sh->hub[e] = ld_agent(&v.deg[e]);
     sync_global();
     if (threadIdx.x == 0) sh->nalive = sh->listlen;
     __syncthreads();
@@ -283,7 +304,8 @@ __device__ void peel_cascade(const ItemView& v, PeelShared<B>* sh, int32_t k) {
                 if (live[r]) live[r] = bit_get(v, u[r]);
             }
 #pragma unroll
-            for (int r = 0; r < U; ++r) old[r] = live[r] ? atomicSub(&v.deg[e[r]], 1u) : 0u;
+            for (int r = 0; r < U; ++r)
+                old[r] = !live[r] ? 0u : e[r] < v.TL ? atomicSub(&sh->hub[e[r]], 1u) : atomicSub(&v.deg[e[r]], 1u);
             int32_t oc[U];
 #pragma unroll
             for (int r = 0; r < U; ++r) oc[r] = (live[r] && old[r] == thr) ? atomicSub(&v.cnt[u[r]], 1) : 0;
@@ -313,8 +335,12 @@ __device__ void peel_scan(const ItemView& v, PeelShared<B>* sh, int32_t k) {
             live = bit_get(v, u);
         }
         uint32_t c = 0;
-        if (live)
-            for (uint32_t l = 0; l < L; ++l) c += ld_agent(&v.deg[uint64_t(u) * L + l]) >= thr;
+        if (live) {
+            if (u < v.T)
+                for (uint32_t l = 0; l < L; ++l) c += sh->hub[u * L + l] >= thr;
+            else
+                for (uint32_t l = 0; l < L; ++l) c += ld_agent(&v.deg[uint64_t(u) * L + l]) >= thr;
+        }
         const bool rem = live && c < lmd;
         const bool keep = live && !rem;
         if (live) v.cnt[u] = int32_t(c);
@@ -331,6 +357,117 @@ __device__ void peel_scan(const ItemView& v, PeelShared<B>* sh, int32_t k) {
     sync_global();  // the cnt stores above feed the cascade's atomics
     if (threadIdx.x == 0) sh->listlen = sh->keep;
     __syncthreads();
+}
+
+// ------------------------------------------------------- batched level 0
+// Level 0 of every item of a graph in one pass over its adjacency, for all
+// the graphs of the batch in one grid-wide launch ahead of the peel: for
+// each vertex v and layer l the counts
+//     deg_lmd[v][l] = #{w in N_l(v) : z(w) >= lmd},   lmd = 1 .. nit,
+// come out of the byte-parallel count of the z stream (hg::count_word), and
+// each item's row and cnt (non-empty rows) are written to its pool slices.
+// Rows are written only for vertices with z(v) >= lmd (the others leave at
+// level 0 and no live entry points at them); cnt = 0 for the others.  A
+// workgroup running item (g, lmd) then only classifies.  Graphs with more
+// than kPullMaxL items keep the per-item pull in peel_level0.
+constexpr uint32_t kPullBlock = 256;
+constexpr uint32_t kPullLightTile = kPullBlock;          // a thread per vertex
+constexpr uint32_t kPullHeavyTile = 2 * kPullBlock / kWave;  // a wavefront per vertex, two per wavefront
+
+struct PullTile {
+    uint32_t g;      // graph
+    uint32_t v0;     // first vertex (internal id)
+    uint32_t count;  // vertices
+    uint32_t heavy;  // 1: ids < Hp, a wavefront per vertex
+};
+
+struct PullItems {
+    uint64_t degb[hg::kPullMaxL];
+    uint64_t sb[hg::kPullMaxL];
+};
+
+template <uint32_t NJ>
+__device__ __forceinline__ void pull_store(const BGraph& gr, const PullItems& it, uint32_t* degpool, uint32_t v,
+                                           uint32_t l, uint32_t zv, const uint32_t* c, uint32_t* cl) {
+#pragma unroll
+    for (uint32_t j = 0; j < NJ; ++j)
+        if (j < gr.nit && j < zv) {
+            degpool[it.degb[j] + uint64_t(v) * gr.L + l] = c[j];
+            cl[j] += c[j] != 0u;
+        }
+}
+
+template <uint32_t NJ>
+__global__ void __launch_bounds__(kPullBlock) k_batch_pull(const BGraph* graphs, const BItem* items_by_id,
+                                                          const PullTile* tiles, uint32_t ntiles, const uint32_t* off,
+                                                          const uint8_t* adjz, const uint8_t* z,
+                                                          const uint32_t* deg0, uint32_t* degpool, int32_t* cntpool) {
+    __shared__ PullItems s_it;
+    const uint32_t lane = lane_id(), wid = threadIdx.x / kWave;
+    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const PullTile tile = tiles[t];
+        const BGraph gr = graphs[tile.g];
+        __syncthreads();  // s_it of the previous tile is no longer read
+        if (threadIdx.x < gr.nit) {
+            const BItem it = items_by_id[gr.item0 + threadIdx.x];
+            s_it.degb[threadIdx.x] = it.degb;
+            s_it.sb[threadIdx.x] = it.sb;
+        }
+        __syncthreads();
+        const uint32_t L = gr.L;
+        if (!tile.heavy) {
+            if (threadIdx.x >= tile.count) continue;
+            const uint32_t v = tile.v0 + threadIdx.x;
+            const uint32_t zv = z[gr.vb + v];
+            uint32_t cl[NJ];
+#pragma unroll
+            for (uint32_t j = 0; j < NJ; ++j) cl[j] = 0;
+            if (zv > 0) {
+                uint64_t pos = gr.eb + off[gr.ob + v];
+                for (uint32_t l = 0; l < L; ++l) {
+                    const uint32_t d = deg0[gr.db + uint64_t(v) * L + l];
+                    uint32_t c[NJ];
+#pragma unroll
+                    for (uint32_t j = 0; j < NJ; ++j) c[j] = 0;
+                    hg::pull_count_seg<NJ>(adjz, pos, pos + d, c);
+                    pos += d;
+                    pull_store<NJ>(gr, s_it, degpool, v, l, zv, c, cl);
+                }
+            }
+#pragma unroll
+            for (uint32_t j = 0; j < NJ; ++j)
+                if (j < gr.nit) cntpool[s_it.sb[j] + v] = j < zv ? int32_t(cl[j]) : 0;
+        } else {
+            for (uint32_t i = wid; i < tile.count; i += kPullBlock / kWave) {
+                const uint32_t v = tile.v0 + i;
+                const uint32_t zv = z[gr.vb + v];
+                uint32_t mine = 0;  // lane j: cnt of lmd = j + 1
+                if (zv > 0) {
+                    uint64_t pos = gr.eb + off[gr.ob + v];
+                    for (uint32_t l = 0; l < L; ++l) {
+                        const uint32_t d = deg0[gr.db + uint64_t(v) * L + l];
+                        uint32_t c[NJ];
+#pragma unroll
+                        for (uint32_t j = 0; j < NJ; ++j) c[j] = 0;
+                        hg::pull_count_vec<NJ>(adjz, pos, pos + d, lane, kWave, c);
+                        pos += d;
+#pragma unroll
+                        for (uint32_t j = 0; j < NJ; ++j)
+                            for (int o = kWave / 2; o > 0; o >>= 1) c[j] += __shfl_xor(c[j], o, kWave);
+                        uint32_t cj = 0;
+#pragma unroll
+                        for (uint32_t j = 0; j < NJ; ++j)
+                            if (lane == j) cj = c[j];
+                        if (lane < gr.nit && lane < zv) {
+                            degpool[s_it.degb[lane] + uint64_t(v) * L + l] = cj;
+                            mine += cj != 0u;
+                        }
+                    }
+                }
+                if (lane < gr.nit) cntpool[s_it.sb[lane] + v] = lane < zv ? int32_t(mine) : 0;
+            }
+        }
+    }
 }
 
 // Resets the item counter (a kernel, so that a captured graph holds no
@@ -373,6 +510,9 @@ __global__ void __launch_bounds__(B) k_batch_peel(const BGraph* graphs, const BI
         v.lmd = item.lmd;
         v.magic = gr.lmagic;
         v.H = gr.H;
+        v.pulled = gr.pulled != 0;
+        v.T = gr.n < hub_words<B>() / gr.L ? gr.n : hub_words<B>() / gr.L;
+        v.TL = v.T * gr.L;
         if (threadIdx.x == 0) {
             sh.listlen = 0;
             sh.qhead = 0;

@@ -34,6 +34,8 @@ def mixed_graphs(count=24, n_hi=6000, base=0xB7):
     gs.append(G.graph(1, [[]]))                                # one vertex
     gs.append(G.graph(5001, [G.star(0, range(1, 5001))] * 2))  # one hub
     gs.append(G.graph(10, [G.clique(range(5)), G.clique(range(5, 10))]))  # disjoint layers
+    # more layers than the batched level-0 pull takes: the item pulls its own rows
+    gs.append(MultilayerGraph.generate(dataclasses.replace(batch_spec(count, base=base, n_lo=200, n_hi=n_hi), L=20)))
     return gs
 
 
@@ -64,6 +66,19 @@ def test_batch_matches_oracle_and_engine():
             assert np.array_equal(kmax, ekmax)
             w = min(cl.shape[1], ecl.shape[1])
             assert np.array_equal(cl[:, :w], ecl[:, :w])
+
+
+def test_batch_few_layers():
+    """Every graph with at most 8 layers (the narrow level-0 counters), hubs
+    of several hundred entries per layer included."""
+    gs = [MultilayerGraph.generate(dataclasses.replace(batch_spec(g, base=0xBA, n_lo=300, n_hi=5000), L=2 + g % 7))
+          for g in range(10)]
+    gs.append(G.graph(3001, [G.star(0, range(1, 3001)), G.star(0, range(1, 1500)), G.star(1, range(2, 700))]))
+    say(f"case batch few layers: {len(gs)} graphs")
+    with Batch(gs) as b:
+        b.decompose()
+        for i, mg in enumerate(gs):
+            assert np.array_equal(b.core_index(i), O.coreindex(mg.n, mg.layers())), f"graph {i}"
 
 
 @pytest.mark.parametrize("block,unroll,bpc,order,use_graph", [(256, 2, 1, 0, 1), (512, 4, 2, 1, 1), (1024, 4, 1, 0, 0),
