@@ -64,7 +64,7 @@ struct hg_batch {
     BItem* d_items = nullptr;  // in queue order
     BItem* d_items_id = nullptr;  // by item id
     PullTile* d_tiles = nullptr;  // level-0 pull work of the pulled graphs
-    uint32_t ntiles = 0, pull_nj = 0;
+    uint32_t ntiles = 0, nheavy = 0, pull_nj = 0;  // tiles [0, nheavy) are heavy
     uint32_t* d_head = nullptr;
     int32_t* d_kmax = nullptr;  // by item id
     uint32_t* degpool = nullptr;
@@ -329,8 +329,21 @@ int create_impl(const hg_batch_graph* gs, int32_t count, void* stream, hg_batch*
         max_pulled = std::max(max_pulled, bg.nit);
         for (uint32_t v = 0; v < bg.Hp; v += kPullHeavyTile)
             tiles.push_back(PullTile{uint32_t(g), v, std::min(kPullHeavyTile, bg.Hp - v), 1u});
-        for (uint32_t v = bg.Hp; v < bg.n; v += kPullLightTile)
-            tiles.push_back(PullTile{uint32_t(g), v, std::min(kPullLightTile, bg.n - v), 0u});
+    }
+    b->nheavy = uint32_t(tiles.size());
+    for (size_t g = 0; g < G; ++g) {
+        const BGraph& bg = b->graphs[g];
+        if (!bg.pulled) continue;
+        // whole vertices, at most kPullBlock rows and kPullStageBytes entries
+        // (a light vertex has at most kPullHeavy entries)
+        const std::vector<int64_t>& o = prep[g].off64;
+        const uint32_t vmax = kPullBlock / bg.L;
+        for (uint32_t v = bg.Hp; v < bg.n;) {
+            uint32_t c = 1;
+            while (v + c < bg.n && c < vmax && uint64_t(o[v + c + 1] - o[v]) <= kPullStageBytes) ++c;
+            tiles.push_back(PullTile{uint32_t(g), v, c, 0u});
+            v += c;
+        }
     }
     b->ntiles = uint32_t(tiles.size());
     b->pull_nj = max_pulled <= 8 ? 8u : 16u;
@@ -527,16 +540,28 @@ int resident_per_cu(int block, int unroll) {
 }
 
 void launch_peel(hg_batch* b) {
-    if (b->ntiles) {
-        const uint32_t grid = std::min<uint32_t>(b->ntiles, uint32_t(b->cus) * 16);
+    const uint32_t cap = uint32_t(b->cus) * 16;
+    const uint32_t nlight = b->ntiles - b->nheavy;
+    const PullTile* light = b->d_tiles + b->nheavy;
+    if (b->nheavy) {
+        const uint32_t grid = std::min(b->nheavy, cap);
         if (b->pull_nj == 8)
-            hipLaunchKernelGGL(k_batch_pull<8>, dim3(grid), dim3(kPullBlock), 0, b->stream, b->d_graphs,
-                               b->d_items_id, b->d_tiles, b->ntiles, b->off, b->adjz, b->z, b->deg0, b->degpool,
+            hipLaunchKernelGGL(k_batch_pull_heavy<8>, dim3(grid), dim3(kPullBlock), 0, b->stream, b->d_graphs,
+                               b->d_items_id, b->d_tiles, b->nheavy, b->off, b->adjz, b->z, b->deg0, b->degpool,
                                b->cntpool);
         else
-            hipLaunchKernelGGL(k_batch_pull<16>, dim3(grid), dim3(kPullBlock), 0, b->stream, b->d_graphs,
-                               b->d_items_id, b->d_tiles, b->ntiles, b->off, b->adjz, b->z, b->deg0, b->degpool,
+            hipLaunchKernelGGL(k_batch_pull_heavy<16>, dim3(grid), dim3(kPullBlock), 0, b->stream, b->d_graphs,
+                               b->d_items_id, b->d_tiles, b->nheavy, b->off, b->adjz, b->z, b->deg0, b->degpool,
                                b->cntpool);
+    }
+    if (nlight) {
+        const uint32_t grid = std::min(nlight, cap);
+        if (b->pull_nj == 8)
+            hipLaunchKernelGGL(k_batch_pull_light<8>, dim3(grid), dim3(kPullBlock), 0, b->stream, b->d_graphs,
+                               b->d_items_id, light, nlight, b->off, b->adjz, b->z, b->deg0, b->degpool, b->cntpool);
+        else
+            hipLaunchKernelGGL(k_batch_pull_light<16>, dim3(grid), dim3(kPullBlock), 0, b->stream, b->d_graphs,
+                               b->d_items_id, light, nlight, b->off, b->adjz, b->z, b->deg0, b->degpool, b->cntpool);
     }
     hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, b->stream, b->d_head);
     const int bpc = std::min(b->params.blocks_per_cu, resident_per_cu(b->params.block, b->params.unroll));

@@ -371,8 +371,10 @@ __device__ void peel_scan(const ItemView& v, PeelShared<B>* sh, int32_t k) {
 // workgroup running item (g, lmd) then only classifies.  Graphs with more
 // than kPullMaxL items keep the per-item pull in peel_level0.
 constexpr uint32_t kPullBlock = 256;
-constexpr uint32_t kPullLightTile = kPullBlock;          // a thread per vertex
 constexpr uint32_t kPullHeavyTile = 2 * kPullBlock / kWave;  // a wavefront per vertex, two per wavefront
+// Light tiles: a thread per (vertex, layer) row, at most kPullBlock rows and
+// kPullStageBytes adjacency bytes (staged in LDS) per tile.
+constexpr uint32_t kPullStageBytes = 8192;
 
 struct PullTile {
     uint32_t g;      // graph
@@ -386,22 +388,14 @@ struct PullItems {
     uint64_t sb[hg::kPullMaxL];
 };
 
+// Heavy head (ids < Hp): a wavefront per vertex, lanes strided over the
+// 16-byte words of each layer segment, counts reduced across the wavefront.
 template <uint32_t NJ>
-__device__ __forceinline__ void pull_store(const BGraph& gr, const PullItems& it, uint32_t* degpool, uint32_t v,
-                                           uint32_t l, uint32_t zv, const uint32_t* c, uint32_t* cl) {
-#pragma unroll
-    for (uint32_t j = 0; j < NJ; ++j)
-        if (j < gr.nit && j < zv) {
-            degpool[it.degb[j] + uint64_t(v) * gr.L + l] = c[j];
-            cl[j] += c[j] != 0u;
-        }
-}
-
-template <uint32_t NJ>
-__global__ void __launch_bounds__(kPullBlock) k_batch_pull(const BGraph* graphs, const BItem* items_by_id,
-                                                          const PullTile* tiles, uint32_t ntiles, const uint32_t* off,
-                                                          const uint8_t* adjz, const uint8_t* z,
-                                                          const uint32_t* deg0, uint32_t* degpool, int32_t* cntpool) {
+__global__ void __launch_bounds__(kPullBlock) k_batch_pull_heavy(const BGraph* graphs, const BItem* items_by_id,
+                                                                const PullTile* tiles, uint32_t ntiles,
+                                                                const uint32_t* off, const uint8_t* adjz,
+                                                                const uint8_t* z, const uint32_t* deg0,
+                                                                uint32_t* degpool, int32_t* cntpool) {
     __shared__ PullItems s_it;
     const uint32_t lane = lane_id(), wid = threadIdx.x / kWave;
     for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
@@ -415,13 +409,10 @@ __global__ void __launch_bounds__(kPullBlock) k_batch_pull(const BGraph* graphs,
         }
         __syncthreads();
         const uint32_t L = gr.L;
-        if (!tile.heavy) {
-            if (threadIdx.x >= tile.count) continue;
-            const uint32_t v = tile.v0 + threadIdx.x;
+        for (uint32_t i = wid; i < tile.count; i += kPullBlock / kWave) {
+            const uint32_t v = tile.v0 + i;
             const uint32_t zv = z[gr.vb + v];
-            uint32_t cl[NJ];
-#pragma unroll
-            for (uint32_t j = 0; j < NJ; ++j) cl[j] = 0;
+            uint32_t mine = 0;  // lane j: cnt of lmd = j + 1
             if (zv > 0) {
                 uint64_t pos = gr.eb + off[gr.ob + v];
                 for (uint32_t l = 0; l < L; ++l) {
@@ -429,43 +420,102 @@ __global__ void __launch_bounds__(kPullBlock) k_batch_pull(const BGraph* graphs,
                     uint32_t c[NJ];
 #pragma unroll
                     for (uint32_t j = 0; j < NJ; ++j) c[j] = 0;
-                    hg::pull_count_seg<NJ>(adjz, pos, pos + d, c);
+                    hg::pull_count_vec<NJ>(adjz, pos, pos + d, lane, kWave, c);
                     pos += d;
-                    pull_store<NJ>(gr, s_it, degpool, v, l, zv, c, cl);
-                }
-            }
 #pragma unroll
-            for (uint32_t j = 0; j < NJ; ++j)
-                if (j < gr.nit) cntpool[s_it.sb[j] + v] = j < zv ? int32_t(cl[j]) : 0;
-        } else {
-            for (uint32_t i = wid; i < tile.count; i += kPullBlock / kWave) {
-                const uint32_t v = tile.v0 + i;
-                const uint32_t zv = z[gr.vb + v];
-                uint32_t mine = 0;  // lane j: cnt of lmd = j + 1
-                if (zv > 0) {
-                    uint64_t pos = gr.eb + off[gr.ob + v];
-                    for (uint32_t l = 0; l < L; ++l) {
-                        const uint32_t d = deg0[gr.db + uint64_t(v) * L + l];
-                        uint32_t c[NJ];
+                    for (uint32_t j = 0; j < NJ; ++j)
+                        for (int o = kWave / 2; o > 0; o >>= 1) c[j] += __shfl_xor(c[j], o, kWave);
+                    uint32_t cj = 0;
 #pragma unroll
-                        for (uint32_t j = 0; j < NJ; ++j) c[j] = 0;
-                        hg::pull_count_vec<NJ>(adjz, pos, pos + d, lane, kWave, c);
-                        pos += d;
-#pragma unroll
-                        for (uint32_t j = 0; j < NJ; ++j)
-                            for (int o = kWave / 2; o > 0; o >>= 1) c[j] += __shfl_xor(c[j], o, kWave);
-                        uint32_t cj = 0;
-#pragma unroll
-                        for (uint32_t j = 0; j < NJ; ++j)
-                            if (lane == j) cj = c[j];
-                        if (lane < gr.nit && lane < zv) {
-                            degpool[s_it.degb[lane] + uint64_t(v) * L + l] = cj;
-                            mine += cj != 0u;
-                        }
+                    for (uint32_t j = 0; j < NJ; ++j)
+                        if (lane == j) cj = c[j];
+                    if (lane < gr.nit && lane < zv) {
+                        degpool[s_it.degb[lane] + uint64_t(v) * L + l] = cj;
+                        mine += cj != 0u;
                     }
                 }
-                if (lane < gr.nit) cntpool[s_it.sb[lane] + v] = lane < zv ? int32_t(mine) : 0;
             }
+            if (lane < gr.nit) cntpool[s_it.sb[lane] + v] = lane < zv ? int32_t(mine) : 0;
+        }
+    }
+}
+
+// The rest: a thread per (vertex, layer) row.  The tile's adjacency bytes
+// are staged in LDS with coalesced word loads and its deg0 rows read
+// coalesced; each thread counts its segment from LDS, the item rows are
+// written coalesced (row r of the tile is word v0 * L + r of every item's
+// rows), and the non-empty row counts are summed per vertex in LDS.
+template <uint32_t NJ>
+__global__ void __launch_bounds__(kPullBlock) k_batch_pull_light(const BGraph* graphs, const BItem* items_by_id,
+                                                                const PullTile* tiles, uint32_t ntiles,
+                                                                const uint32_t* off, const uint8_t* adjz,
+                                                                const uint8_t* z, const uint32_t* deg0,
+                                                                uint32_t* degpool, int32_t* cntpool) {
+    __shared__ PullItems s_it;
+    __shared__ uint32_t s_stage[kPullStageBytes / 4 + 2];
+    __shared__ uint32_t s_cl[kPullBlock * (NJ / 4)];  // per vertex of the tile: packed non-empty row counts
+    __shared__ uint32_t s_tmp[kPullBlock / kWave + 1];
+    const uint8_t* stage = reinterpret_cast<const uint8_t*>(s_stage);
+    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const PullTile tile = tiles[t];
+        const BGraph gr = graphs[tile.g];
+        const uint32_t L = gr.L, rows = tile.count * L;
+        __syncthreads();  // LDS of the previous tile is no longer read
+        if (threadIdx.x < gr.nit) {
+            const BItem it = items_by_id[gr.item0 + threadIdx.x];
+            s_it.degb[threadIdx.x] = it.degb;
+            s_it.sb[threadIdx.x] = it.sb;
+        }
+        for (uint32_t i = threadIdx.x; i < tile.count * (NJ / 4); i += kPullBlock) s_cl[i] = 0;
+        // the tile's bytes [beg, end) of adjz, from the aligned word below beg
+        const uint64_t beg = gr.eb + off[gr.ob + tile.v0], end = gr.eb + off[gr.ob + tile.v0 + tile.count];
+        const uint64_t w0 = beg >> 2;
+        const uint32_t nw = uint32_t(((end + 3) >> 2) - w0);  // <= kPullStageBytes / 4 + 1
+        const uint32_t* zw = reinterpret_cast<const uint32_t*>(adjz);
+        for (uint32_t i = threadIdx.x; i < nw; i += kPullBlock) s_stage[i] = zw[w0 + i];
+        // this thread's row: its length and start (block scan of the lengths)
+        const uint32_t r = threadIdx.x;
+        const uint32_t d = r < rows ? deg0[gr.db + uint64_t(tile.v0) * L + r] : 0u;
+        uint32_t total;
+        const uint32_t pre = bscan<kPullBlock>(d, &total, s_tmp);  // ends with a barrier
+        if (r < rows) {
+            const uint32_t vl = r / L;
+            const uint32_t v = tile.v0 + vl;
+            const uint32_t zv = z[gr.vb + v];
+            uint32_t c4[NJ / 4];
+#pragma unroll
+            for (uint32_t q = 0; q < NJ / 4; ++q) c4[q] = 0;
+            const uint32_t s0 = uint32_t(beg - (w0 << 2)) + pre;
+            for (uint32_t k = 0; k < d; ++k) {
+                const int32_t zb = stage[s0 + k];  // entries with z > j for j < zb
+#pragma unroll
+                for (uint32_t q = 0; q < NJ / 4; ++q) {
+                    const int32_t m = zb - int32_t(4 * q);
+                    c4[q] += m >= 4 ? 0x01010101u : m <= 0 ? 0u : 0x01010101u & ((1u << (8 * m)) - 1u);
+                }
+            }
+            const uint64_t row = uint64_t(tile.v0) * L + r;
+#pragma unroll
+            for (uint32_t j = 0; j < NJ; ++j)
+                if (j < gr.nit && j < zv) degpool[s_it.degb[j] + row] = (c4[j >> 2] >> (8 * (j & 3))) & 0xFFu;
+            // non-empty rows: byte j of word q is 1 when count j is non-zero
+#pragma unroll
+            for (uint32_t q = 0; q < NJ / 4; ++q) {
+                uint32_t nz = 0;
+#pragma unroll
+                for (uint32_t b = 0; b < 4; ++b) nz |= uint32_t(((c4[q] >> (8 * b)) & 0xFFu) != 0u) << (8 * b);
+                if (nz) atomicAdd(&s_cl[vl * (NJ / 4) + q], nz);
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x < tile.count) {
+            const uint32_t v = tile.v0 + threadIdx.x;
+            const uint32_t zv = z[gr.vb + v];
+#pragma unroll
+            for (uint32_t j = 0; j < NJ; ++j)
+                if (j < gr.nit)
+                    cntpool[s_it.sb[j] + v] =
+                        j < zv ? int32_t((s_cl[threadIdx.x * (NJ / 4) + (j >> 2)] >> (8 * (j & 3))) & 0xFFu) : 0;
         }
     }
 }

@@ -74,6 +74,10 @@ def test_batch_few_layers():
     gs = [MultilayerGraph.generate(dataclasses.replace(batch_spec(g, base=0xBA, n_lo=300, n_hi=5000), L=2 + g % 7))
           for g in range(10)]
     gs.append(G.graph(3001, [G.star(0, range(1, 3001)), G.star(0, range(1, 1500)), G.star(1, range(2, 700))]))
+    # 20 layers of which 5 hold edges: few items, many rows per vertex
+    rng = np.random.default_rng(0xBB)
+    gs.append(G.graph(2000, [[tuple(e) for e in rng.integers(0, 2000, size=(6000, 2))] if l % 4 == 0 else []
+                             for l in range(20)]))
     say(f"case batch few layers: {len(gs)} graphs")
     with Batch(gs) as b:
         b.decompose()
