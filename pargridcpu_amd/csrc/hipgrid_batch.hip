@@ -293,6 +293,7 @@ int create_impl(const hg_batch_graph* gs, int32_t count, void* stream, hg_batch*
         bg.item0 = uint32_t(b->items.size());
         bg.nit = std::min(b->L[g], b->max_nz[g]);
         bg.pulled = bg.nit >= 1 && bg.nit <= hg::kPullMaxL;
+        bg.hub16 = prep[g].maxdeg < (1u << 16);
         b->item_of[g].assign(b->L[g], -1);
         for (uint32_t lmd = 1; lmd <= b->L[g] && lmd <= b->max_nz[g]; ++lmd) {
             BItem it{};

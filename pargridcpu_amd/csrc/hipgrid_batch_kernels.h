@@ -61,6 +61,8 @@ struct BGraph {
     uint32_t item0, nit;       // the graph's items: ids item0 .. item0 + nit - 1, lmd = 1 .. nit
     uint32_t pulled;           // 1: k_batch_pull writes level 0 of every item (nit <= kPullMaxL)
     uint32_t Hp;               // vertices with more than kPullHeavy entries (ids < Hp)
+    uint32_t hub16;            // 1: every layer degree < 2^16, LDS hub rows hold two degrees a word
+    uint32_t pad;
 };
 
 struct BItem {
@@ -110,12 +112,16 @@ __device__ __forceinline__ void sync_global() {
 }
 
 
-// Degree words of the item's heaviest vertices (internal ids < T, T * L <=
-// the table) live in LDS for the whole chain: their decrements are LDS
-// atomics instead of device-scope ones.  Ids are sorted by total degree, so
-// the table holds the rows that take the most decrements (measured on
-// graphs of 90k-180k vertices: 17-32 % of all decrements with 96 KB, most
-// of them for graphs of a few ten thousand vertices).
+// Degree words of the item's heaviest vertices (internal ids < T) live in
+// LDS for the whole chain: their decrements are LDS atomics instead of
+// device-scope ones.  Ids are sorted by total degree, so the table holds the
+// rows that take the most decrements.  When every layer degree of the graph
+// is below 2^16 a table word holds two degrees (entry e in half e & 1 of
+// word e >> 1; a live decrement never takes a degree below zero, so no
+// borrow crosses the halves) and T doubles.  Measured on graphs of
+// 90k-180k vertices, a 96 KB table takes 17-32 % of all decrements with one
+// degree a word and 24-40 % with two, most of them for graphs of a few ten
+// thousand vertices.
 template <int B>
 constexpr uint32_t hub_words() {
     return B >= 1024 ? 24576u : B >= 512 ? 8192u : 4096u;
@@ -153,8 +159,21 @@ struct ItemView {
     bool lds_bits;
     bool pulled;           // level-0 rows and cnt written by k_batch_pull
     uint32_t n, L, lmd, magic, H;
-    uint32_t T, TL;        // vertices < T have their degree rows in LDS (words < TL = T * L)
+    uint32_t T, TL;        // vertices < T have their degree rows in LDS (entries < TL = T * L)
+    bool hub16;            // two degrees a table word
 };
+
+// LDS hub table: degree of entry e < TL (reads), decrement returning the old degree.
+template <int B>
+__device__ __forceinline__ uint32_t hub_get(const ItemView& v, const PeelShared<B>* sh, uint32_t e) {
+    return v.hub16 ? (sh->hub[e >> 1] >> (16 * (e & 1))) & 0xFFFFu : sh->hub[e];
+}
+template <int B>
+__device__ __forceinline__ uint32_t hub_dec(const ItemView& v, PeelShared<B>* sh, uint32_t e) {
+    if (!v.hub16) return atomicSub(&sh->hub[e], 1u);
+    const uint32_t sh16 = 16 * (e & 1);
+    return (atomicSub(&sh->hub[e >> 1], 1u << sh16) >> sh16) & 0xFFFFu;
+}
 
 __device__ __forceinline__ bool bit_get(const ItemView& v, uint32_t u) {
     const uint32_t w = v.lds_bits ? v.bits[u >> 5] : __hip_atomic_load(&v.bits[u >> 5], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -250,7 +269,15 @@ __device__ void peel_level0(const ItemView& v, PeelShared<B>* sh) {
         }
     }
     // the hub rows move to LDS (rows of vertices with z < lmd are never read)
-    for (uint32_t e = threadIdx.x; e < v.TL; e += B) sh->hub[e] = ld_agent(&v.deg[e]);
+    if (v.hub16) {
+        for (uint32_t w = threadIdx.x; 2 * w < v.TL; w += B) {
+            const uint32_t lo = ld_agent(&v.deg[2 * w]);
+            const uint32_t hi = 2 * w + 1 < v.TL ? ld_agent(&v.deg[2 * w + 1]) : 0u;
+            sh->hub[w] = lo | (hi << 16);
+        }
+    } else {
+        for (uint32_t e = threadIdx.x; e < v.TL; e += B) sh->hub[e] = ld_agent(&v.deg[e]);
+    }
     sync_global();
     if (threadIdx.x == 0) sh->nalive = sh->listlen;
     __syncthreads();
@@ -305,7 +332,7 @@ __device__ void peel_cascade(const ItemView& v, PeelShared<B>* sh, int32_t k) {
             }
 #pragma unroll
             for (int r = 0; r < U; ++r)
-                old[r] = !live[r] ? 0u : e[r] < v.TL ? atomicSub(&sh->hub[e[r]], 1u) : atomicSub(&v.deg[e[r]], 1u);
+                old[r] = !live[r] ? 0u : e[r] < v.TL ? hub_dec<B>(v, sh, e[r]) : atomicSub(&v.deg[e[r]], 1u);
             int32_t oc[U];
 #pragma unroll
             for (int r = 0; r < U; ++r) oc[r] = (live[r] && old[r] == thr) ? atomicSub(&v.cnt[u[r]], 1) : 0;
@@ -337,7 +364,7 @@ __device__ void peel_scan(const ItemView& v, PeelShared<B>* sh, int32_t k) {
         uint32_t c = 0;
         if (live) {
             if (u < v.T)
-                for (uint32_t l = 0; l < L; ++l) c += sh->hub[u * L + l] >= thr;
+                for (uint32_t l = 0; l < L; ++l) c += hub_get<B>(v, sh, u * L + l) >= thr;
             else
                 for (uint32_t l = 0; l < L; ++l) c += ld_agent(&v.deg[uint64_t(u) * L + l]) >= thr;
         }
@@ -561,7 +588,11 @@ __global__ void __launch_bounds__(B) k_batch_peel(const BGraph* graphs, const BI
         v.magic = gr.lmagic;
         v.H = gr.H;
         v.pulled = gr.pulled != 0;
-        v.T = gr.n < hub_words<B>() / gr.L ? gr.n : hub_words<B>() / gr.L;
+        v.hub16 = gr.hub16 != 0;
+        {
+            const uint32_t cap = (v.hub16 ? 2u : 1u) * hub_words<B>() / gr.L;
+            v.T = gr.n < cap ? gr.n : cap;
+        }
         v.TL = v.T * gr.L;
         if (threadIdx.x == 0) {
             sh.listlen = 0;

@@ -78,6 +78,10 @@ def test_batch_few_layers():
     rng = np.random.default_rng(0xBB)
     gs.append(G.graph(2000, [[tuple(e) for e in rng.integers(0, 2000, size=(6000, 2))] if l % 4 == 0 else []
                              for l in range(20)]))
+    # a layer degree of 2^16 or more: one degree per LDS hub word
+    big = 70_001
+    gs.append(G.graph(big, [G.star(0, range(1, big)) + [tuple(e) for e in rng.integers(0, big, size=(50_000, 2))],
+                            G.cycle(range(big)) + G.clique(range(40))]))
     say(f"case batch few layers: {len(gs)} graphs")
     with Batch(gs) as b:
         b.decompose()
