@@ -115,7 +115,9 @@ __device__ __forceinline__ void sync_global() {
 // Degree words of the item's heaviest vertices (internal ids < T) live in
 // LDS for the whole chain: their decrements are LDS atomics instead of
 // device-scope ones.  Ids are sorted by total degree, so the table holds the
-// rows that take the most decrements.  When every layer degree of the graph
+// rows that take the most decrements.  The table fills what LDS is left:
+// 118 KB for 1024-thread workgroups (one per CU), 32 KB for 512 and 16 KB
+// for 256 (two and three per CU).  When every layer degree of the graph
 // is below 2^16 a table word holds two degrees (entry e in half e & 1 of
 // word e >> 1; a live decrement never takes a degree below zero, so no
 // borrow crosses the halves) and T doubles.  Measured on graphs of
@@ -124,7 +126,7 @@ __device__ __forceinline__ void sync_global() {
 // thousand vertices.
 template <int B>
 constexpr uint32_t hub_words() {
-    return B >= 1024 ? 24576u : B >= 512 ? 8192u : 4096u;
+    return B >= 1024 ? 30208u : B >= 512 ? 8192u : 4096u;
 }
 
 template <int B>
