@@ -154,7 +154,7 @@ def run_batched(steps: int, count: int = 256, params: str = "tune") -> dict:
                             else (None, None))
     roof = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic, "traffic_source": traffic_src,
-            "kernel": "k_batch_peel (one launch)", "algorithmic_bytes": balg, "avg_ms": round(dev_ms, 4)}
+            "kernel": "k_batch_pull_heavy + k_batch_pull_light + k_batch_peel (one launch)", "algorithmic_bytes": balg, "avg_ms": round(dev_ms, 4)}
     log(f"[bench] batched: launch {batch_ms:.2f} ms wall ({dev_ms:.2f} ms device), sequential {seq_ms:.1f} ms, "
         f"exact={exact}")
     return {"workload": f"{count} graphs: n log-U[1e3,2e5], L U{{2..16}}, avg deg U[2,20], gamma U[2.1,3.0]",
