@@ -59,6 +59,25 @@ def test_seeded_small(n, L, m, seed):
     assert np.array_equal(cl, pcl)
 
 
+@pytest.mark.parametrize("heavy,blocks", [(4, 0), (4, 5), (64, 3)])
+def test_deferred_chunks(monkeypatch, heavy, blocks):
+    """With a tiny heavy threshold almost every removal is deferred to the
+    next step as chunks (a small grid gives each block many chunks of every
+    instance); the results stay exact, also on a second decomposition of the
+    same graph."""
+    monkeypatch.setenv("HG_HEAVY", str(heavy))
+    if blocks:
+        monkeypatch.setenv("HG_BLOCKS", str(blocks))
+    for n, L, m, seed in [(3000, 5, 12000, 11), (1500, 8, 9000, 12), (800, 3, 30000, 13)]:
+        mg = MultilayerGraph.generate(small_spec(n, L, m, seed))
+        say(f"case deferred heavy={heavy} blocks={blocks} n={n} L={L} m={m} seed={seed}")
+        ref = O.coreindex(mg.n, mg.layers())
+        with Engine.from_graph(mg) as eng:
+            for rep in range(2):
+                eng.decompose()
+                assert np.array_equal(eng.core_index(), ref), f"repeat {rep}"
+
+
 def test_c1_workload():
     mg = MultilayerGraph.generate(C1)
     say("case C1 n=20000 L=7")
