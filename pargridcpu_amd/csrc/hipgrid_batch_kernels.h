@@ -132,8 +132,9 @@ constexpr uint32_t hub_words() {
 
 template <int B>
 struct PeelShared {
-    uint32_t hub[hub_words<B>()];
-    uint32_t bits[kLdsWords];
+    // the liveness bitmap (its first ceil(n / 32) words, when n <= kLdsBits),
+    // then the hub table in the rest
+    uint32_t pool[kLdsWords + hub_words<B>()];
     uint32_t pre[B];
     uint32_t start[B];
     uint32_t tmp[B / kWave + 1];
@@ -163,19 +164,22 @@ struct ItemView {
     bool pulled;           // level-0 rows and cnt written by k_batch_pull
     uint32_t n, L, lmd, magic, H;
     uint32_t T, TL;        // vertices < T have their degree rows in LDS (entries < TL = T * L)
+    uint32_t hub0;         // first pool word of the hub table
     bool hub16;            // two degrees a table word
 };
 
 // LDS hub table: degree of entry e < TL (reads), decrement returning the old degree.
 template <int B>
 __device__ __forceinline__ uint32_t hub_get(const ItemView& v, const PeelShared<B>* sh, uint32_t e) {
-    return v.hub16 ? (sh->hub[e >> 1] >> (16 * (e & 1))) & 0xFFFFu : sh->hub[e];
+    const uint32_t* hub = sh->pool + v.hub0;
+    return v.hub16 ? (hub[e >> 1] >> (16 * (e & 1))) & 0xFFFFu : hub[e];
 }
 template <int B>
 __device__ __forceinline__ uint32_t hub_dec(const ItemView& v, PeelShared<B>* sh, uint32_t e) {
-    if (!v.hub16) return atomicSub(&sh->hub[e], 1u);
+    uint32_t* hub = sh->pool + v.hub0;
+    if (!v.hub16) return atomicSub(&hub[e], 1u);
     const uint32_t sh16 = 16 * (e & 1);
-    return (atomicSub(&sh->hub[e >> 1], 1u << sh16) >> sh16) & 0xFFFFu;
+    return (atomicSub(&hub[e >> 1], 1u << sh16) >> sh16) & 0xFFFFu;
 }
 
 __device__ __forceinline__ bool bit_get(const ItemView& v, uint32_t u) {
@@ -276,10 +280,10 @@ __device__ void peel_level0(const ItemView& v, PeelShared<B>* sh) {
         for (uint32_t w = threadIdx.x; 2 * w < v.TL; w += B) {
             const uint32_t lo = ld_agent(&v.deg[2 * w]);
             const uint32_t hi = 2 * w + 1 < v.TL ? ld_agent(&v.deg[2 * w + 1]) : 0u;
-            sh->hub[w] = lo | (hi << 16);
+            sh->pool[v.hub0 + w] = lo | (hi << 16);
         }
     } else {
-        for (uint32_t e = threadIdx.x; e < v.TL; e += B) sh->hub[e] = ld_agent(&v.deg[e]);
+        for (uint32_t e = threadIdx.x; e < v.TL; e += B) sh->pool[v.hub0 + e] = ld_agent(&v.deg[e]);
     }
     sync_global();
     if (threadIdx.x == 0) sh->nalive = sh->listlen;
@@ -584,7 +588,7 @@ __global__ void __launch_bounds__(B) k_batch_peel(const BGraph* graphs, const BI
         v.list = listpool + item.sb;
         v.queue = qpool + item.sb;
         v.lds_bits = gr.n <= kLdsBits;
-        v.bits = v.lds_bits ? sh.bits : bitpool + item.bitb;
+        v.bits = v.lds_bits ? sh.pool : bitpool + item.bitb;
         v.n = gr.n;
         v.L = gr.L;
         v.lmd = item.lmd;
@@ -593,7 +597,9 @@ __global__ void __launch_bounds__(B) k_batch_peel(const BGraph* graphs, const BI
         v.pulled = gr.pulled != 0;
         v.hub16 = gr.hub16 != 0;
         {
-            const uint32_t cap = (v.hub16 ? 2u : 1u) * hub_words<B>() / gr.L;
+            // the bitmap words in LDS (rounded to 16 bytes), the table after them
+            v.hub0 = v.lds_bits ? ((gr.n + 31) / 32 + 3) & ~3u : 0u;
+            const uint32_t cap = (v.hub16 ? 2u : 1u) * (kLdsWords + hub_words<B>() - v.hub0) / gr.L;
             v.T = gr.n < cap ? gr.n : cap;
         }
         v.TL = v.T * gr.L;
