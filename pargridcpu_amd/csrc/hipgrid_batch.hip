@@ -64,7 +64,8 @@ struct hg_batch {
     BItem* d_items = nullptr;  // in queue order
     BItem* d_items_id = nullptr;  // by item id
     PullTile* d_tiles = nullptr;  // level-0 pull work of the pulled graphs
-    uint32_t ntiles = 0, nheavy = 0, pull_nj = 0;  // tiles [0, nheavy) are heavy
+    // pull tiles: [0, nheavy) hubs, [nheavy, nmed) medium vertices, the rest light
+    uint32_t ntiles = 0, nheavy = 0, nmed = 0, pull_nj = 0;
     uint32_t* d_head = nullptr;
     int32_t* d_kmax = nullptr;  // by item id
     uint32_t* degpool = nullptr;
@@ -137,7 +138,7 @@ struct Prep {
     std::vector<uint8_t> zc, zr;           // z in caller's / internal ids
     std::vector<int64_t> off64;            // merged offsets, internal ids
     std::vector<uint64_t> ent_z;           // [z]: entries of the vertices with z layers, then suffix sums
-    uint32_t H = 0, Hp = 0, max_nz = 0;
+    uint32_t H = 0, Hp = 0, Hh = 0, max_nz = 0;
     uint64_t D = 0, maxdeg = 0;
 };
 
@@ -209,6 +210,7 @@ int prepare(const hg_batch_graph& gr, int32_t gi, Prep& P) {
         P.zr[size_t(r)] = P.zc[v];
         if (tot[v] > kWaveVertex) P.H = uint32_t(r) + 1;
         if (tot[v] > hg::kPullHeavy) P.Hp = uint32_t(r) + 1;
+        if (tot[v] > kPullStageBytes) P.Hh = uint32_t(r) + 1;
     }
     P.D = uint64_t(P.off64[size_t(n)]);
     if (P.D >= (1ull << 32)) return fail(HG_ERR_RANGE, tag + "directed entries must be < 2^32");
@@ -328,24 +330,27 @@ int create_impl(const hg_batch_graph* gs, int32_t count, void* stream, hg_batch*
         const BGraph& bg = b->graphs[g];
         if (!bg.pulled) continue;
         max_pulled = std::max(max_pulled, bg.nit);
-        for (uint32_t v = 0; v < bg.Hp; v += kPullHeavyTile)
-            tiles.push_back(PullTile{uint32_t(g), v, std::min(kPullHeavyTile, bg.Hp - v), 1u});
+        for (uint32_t v = 0; v < prep[g].Hh; v += kPullHeavyTile)
+            tiles.push_back(PullTile{uint32_t(g), v, std::min(kPullHeavyTile, prep[g].Hh - v), 1u});
     }
     b->nheavy = uint32_t(tiles.size());
-    for (size_t g = 0; g < G; ++g) {
-        const BGraph& bg = b->graphs[g];
-        if (!bg.pulled) continue;
-        // whole vertices, at most kPullBlock rows and kPullStageBytes entries
-        // (a light vertex has at most kPullHeavy entries)
+    // whole vertices, at most kPullBlock rows and kPullStageBytes entries
+    // (a vertex below Hh has at most kPullStageBytes entries)
+    auto row_tiles = [&](size_t g, uint32_t v0, uint32_t v1) {
         const std::vector<int64_t>& o = prep[g].off64;
-        const uint32_t vmax = kPullBlock / bg.L;
-        for (uint32_t v = bg.Hp; v < bg.n;) {
+        const uint32_t vmax = kPullBlock / b->graphs[g].L;
+        for (uint32_t v = v0; v < v1;) {
             uint32_t c = 1;
-            while (v + c < bg.n && c < vmax && uint64_t(o[v + c + 1] - o[v]) <= kPullStageBytes) ++c;
+            while (v + c < v1 && c < vmax && uint64_t(o[v + c + 1] - o[v]) <= kPullStageBytes) ++c;
             tiles.push_back(PullTile{uint32_t(g), v, c, 0u});
             v += c;
         }
-    }
+    };
+    for (size_t g = 0; g < G; ++g)
+        if (b->graphs[g].pulled) row_tiles(g, prep[g].Hh, b->graphs[g].Hp);
+    b->nmed = uint32_t(tiles.size());
+    for (size_t g = 0; g < G; ++g)
+        if (b->graphs[g].pulled) row_tiles(g, b->graphs[g].Hp, b->graphs[g].n);
     b->ntiles = uint32_t(tiles.size());
     b->pull_nj = max_pulled <= 8 ? 8u : 16u;
 
@@ -542,8 +547,9 @@ int resident_per_cu(int block, int unroll) {
 
 void launch_peel(hg_batch* b) {
     const uint32_t cap = uint32_t(b->cus) * 16;
-    const uint32_t nlight = b->ntiles - b->nheavy;
-    const PullTile* light = b->d_tiles + b->nheavy;
+    const uint32_t nmed = b->nmed - b->nheavy, nlight = b->ntiles - b->nmed;
+    const PullTile* med = b->d_tiles + b->nheavy;
+    const PullTile* light = b->d_tiles + b->nmed;
     if (b->nheavy) {
         const uint32_t grid = std::min(b->nheavy, cap);
         if (b->pull_nj == 8)
@@ -555,13 +561,22 @@ void launch_peel(hg_batch* b) {
                                b->d_items_id, b->d_tiles, b->nheavy, b->off, b->adjz, b->z, b->deg0, b->degpool,
                                b->cntpool);
     }
+    if (nmed) {
+        const uint32_t grid = std::min(nmed, cap);
+        if (b->pull_nj == 8)
+            hipLaunchKernelGGL((k_batch_pull_light<8, 16>), dim3(grid), dim3(kPullBlock), 0, b->stream, b->d_graphs,
+                               b->d_items_id, med, nmed, b->off, b->adjz, b->z, b->deg0, b->degpool, b->cntpool);
+        else
+            hipLaunchKernelGGL((k_batch_pull_light<16, 16>), dim3(grid), dim3(kPullBlock), 0, b->stream, b->d_graphs,
+                               b->d_items_id, med, nmed, b->off, b->adjz, b->z, b->deg0, b->degpool, b->cntpool);
+    }
     if (nlight) {
         const uint32_t grid = std::min(nlight, cap);
         if (b->pull_nj == 8)
-            hipLaunchKernelGGL(k_batch_pull_light<8>, dim3(grid), dim3(kPullBlock), 0, b->stream, b->d_graphs,
+            hipLaunchKernelGGL((k_batch_pull_light<8, 8>), dim3(grid), dim3(kPullBlock), 0, b->stream, b->d_graphs,
                                b->d_items_id, light, nlight, b->off, b->adjz, b->z, b->deg0, b->degpool, b->cntpool);
         else
-            hipLaunchKernelGGL(k_batch_pull_light<16>, dim3(grid), dim3(kPullBlock), 0, b->stream, b->d_graphs,
+            hipLaunchKernelGGL((k_batch_pull_light<16, 8>), dim3(grid), dim3(kPullBlock), 0, b->stream, b->d_graphs,
                                b->d_items_id, light, nlight, b->off, b->adjz, b->z, b->deg0, b->degpool, b->cntpool);
     }
     hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, b->stream, b->d_head);

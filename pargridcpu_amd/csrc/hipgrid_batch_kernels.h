@@ -99,17 +99,19 @@ __device__ __forceinline__ uint32_t bscan(uint32_t x, uint32_t* total, uint32_t*
     return r;
 }
 
-// Phase boundaries inside an item.  The item's degree, count, list and
-// queue words are written with plain stores by one wave and then read with
-// device-scope loads or updated with device-scope atomics by other waves of
-// the same workgroup.  All of them run on one CU and reach memory through
-// that CU's L2, so waiting until the stores are complete and then meeting
-// at the barrier (hg::sync_stores) is enough.  A device-scope fence
-// (__threadfence) would in addition write back the whole L2 of the XCD
-// (buffer_wbl2) at every boundary: measured on the 256-graph batch, those
-// write-backs took 7 % of the launch (139 vs 130 ms).
+// Block barrier that also publishes this thread's global-memory writes at
+// device scope.  The item's degree and count words are written with plain
+// stores by one wave and then updated with device-scope atomics, or read
+// with device-scope loads, by others in the same launch; a plain barrier
+// orders them at workgroup scope only, which does not make a store visible
+// to those device-scope operations.  Used at every phase boundary where
+// one wave's stores feed another wave's atomics or loads.  (Waiting for the
+// stores alone, without the fence's L2 write-back, made the launch 7 %
+// faster but gave wrong indices in 1 of ~360 batches run from four threads
+// at once.)
 __device__ __forceinline__ void sync_global() {
-    sync_stores();
+    __threadfence();
+    __syncthreads();
 }
 
 
@@ -479,7 +481,9 @@ __global__ void __launch_bounds__(kPullBlock) k_batch_pull_heavy(const BGraph* g
 // coalesced; each thread counts its segment from LDS, the item rows are
 // written coalesced (row r of the tile is word v0 * L + r of every item's
 // rows), and the non-empty row counts are summed per vertex in LDS.
-template <uint32_t NJ>
+// W = 8: vertices with at most kPullHeavy entries, counts packed four to a
+// word; W = 16: vertices with at most kPullStageBytes entries, two to a word.
+template <uint32_t NJ, uint32_t W>
 __global__ void __launch_bounds__(kPullBlock) k_batch_pull_light(const BGraph* graphs, const BItem* items_by_id,
                                                                 const PullTile* tiles, uint32_t ntiles,
                                                                 const uint32_t* off, const uint8_t* adjz,
@@ -516,28 +520,34 @@ __global__ void __launch_bounds__(kPullBlock) k_batch_pull_light(const BGraph* g
             const uint32_t vl = r / L;
             const uint32_t v = tile.v0 + vl;
             const uint32_t zv = z[gr.vb + v];
-            uint32_t c4[NJ / 4];
+            constexpr uint32_t P = 32 / W;  // counts a word
+            constexpr uint32_t M = (1u << W) - 1u;
+            constexpr uint32_t ONES = W == 8 ? 0x01010101u : 0x00010001u;
+            uint32_t cw[NJ / P];
 #pragma unroll
-            for (uint32_t q = 0; q < NJ / 4; ++q) c4[q] = 0;
+            for (uint32_t q = 0; q < NJ / P; ++q) cw[q] = 0;
             const uint32_t s0 = uint32_t(beg - (w0 << 2)) + pre;
             for (uint32_t k = 0; k < d; ++k) {
                 const int32_t zb = stage[s0 + k];  // entries with z > j for j < zb
 #pragma unroll
-                for (uint32_t q = 0; q < NJ / 4; ++q) {
-                    const int32_t m = zb - int32_t(4 * q);
-                    c4[q] += m >= 4 ? 0x01010101u : m <= 0 ? 0u : 0x01010101u & ((1u << (8 * m)) - 1u);
+                for (uint32_t q = 0; q < NJ / P; ++q) {
+                    const int32_t m = zb - int32_t(P * q);  // counts of word q that this entry raises
+                    cw[q] += m >= int32_t(P) ? ONES : m <= 0 ? 0u : ONES & ((1u << (W * m)) - 1u);
                 }
             }
             const uint64_t row = uint64_t(tile.v0) * L + r;
 #pragma unroll
             for (uint32_t j = 0; j < NJ; ++j)
-                if (j < gr.nit && j < zv) degpool[s_it.degb[j] + row] = (c4[j >> 2] >> (8 * (j & 3))) & 0xFFu;
+                if (j < gr.nit && j < zv) degpool[s_it.degb[j] + row] = (cw[j / P] >> (W * (j % P))) & M;
             // non-empty rows: byte j of word q is 1 when count j is non-zero
 #pragma unroll
             for (uint32_t q = 0; q < NJ / 4; ++q) {
                 uint32_t nz = 0;
 #pragma unroll
-                for (uint32_t b = 0; b < 4; ++b) nz |= uint32_t(((c4[q] >> (8 * b)) & 0xFFu) != 0u) << (8 * b);
+                for (uint32_t b = 0; b < 4; ++b) {
+                    const uint32_t j = 4 * q + b;
+                    nz |= uint32_t(((cw[j / P] >> (W * (j % P))) & M) != 0u) << (8 * b);
+                }
                 if (nz) atomicAdd(&s_cl[vl * (NJ / 4) + q], nz);
             }
         }
