@@ -120,7 +120,8 @@ __device__ __forceinline__ void sync_global() {
 // device-scope ones.  Ids are sorted by total degree, so the table holds the
 // rows that take the most decrements.  The table fills what LDS is left:
 // 118 KB for 1024-thread workgroups (one per CU), 32 KB for 512 and 16 KB
-// for 256 (two and three per CU).  When every layer degree of the graph
+// for 256 (two and three per CU), plus the part of the 32 KB bitmap region
+// that the item's bitmap leaves free.  When every layer degree of the graph
 // is below 2^16 a table word holds two degrees (entry e in half e & 1 of
 // word e >> 1; a live decrement never takes a degree below zero, so no
 // borrow crosses the halves) and T doubles.  Measured on graphs of
