@@ -376,6 +376,14 @@ int create_impl(const hg_batch_graph* gs, int32_t count, void* stream, hg_batch*
     HIP_TRY(dalloc(&b->qpool, SN)); acct(4 * SN);
     HIP_TRY(dalloc(&b->bitpool, SW)); acct(4 * SW);
     b->device_bytes = bytes;
+    if (env_int("HG_POISON", 0) == 1) {  // diagnostic: item state starts as all-ones words, not zeros
+        HIP_TRY(hipMemsetAsync(b->degpool, 0xFF, 4 * std::max<uint64_t>(SDEG, 1), b->stream));
+        HIP_TRY(hipMemsetAsync(b->cntpool, 0xFF, 4 * std::max<uint64_t>(SN, 1), b->stream));
+        HIP_TRY(hipMemsetAsync(b->idxpool, 0xFF, 4 * std::max<uint64_t>(SN, 1), b->stream));
+        HIP_TRY(hipMemsetAsync(b->listpool, 0xFF, 4 * std::max<uint64_t>(SN, 1), b->stream));
+        HIP_TRY(hipMemsetAsync(b->qpool, 0xFF, 4 * std::max<uint64_t>(SN, 1), b->stream));
+        HIP_TRY(hipMemsetAsync(b->bitpool, 0xFF, 4 * std::max<uint64_t>(SW, 1), b->stream));
+    }
 
     // Upload and build: per-layer CSR into temporaries, then per graph the
     // single-graph engine's set-up kernels write deg0 and the merged,

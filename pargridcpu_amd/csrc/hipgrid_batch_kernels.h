@@ -280,9 +280,13 @@ __device__ void peel_level0(const ItemView& v, PeelShared<B>* sh) {
     }
     // the hub rows move to LDS (rows of vertices with z < lmd are never read)
     if (v.hub16) {
+        // A word can pair the last row of one vertex with the first of the
+        // next (odd L), and the rows of a vertex with z < lmd hold whatever
+        // the pool held before: each half is cut to 16 bits so that such a
+        // word never spills into its valid neighbour.
         for (uint32_t w = threadIdx.x; 2 * w < v.TL; w += B) {
-            const uint32_t lo = ld_agent(&v.deg[2 * w]);
-            const uint32_t hi = 2 * w + 1 < v.TL ? ld_agent(&v.deg[2 * w + 1]) : 0u;
+            const uint32_t lo = ld_agent(&v.deg[2 * w]) & 0xFFFFu;
+            const uint32_t hi = 2 * w + 1 < v.TL ? ld_agent(&v.deg[2 * w + 1]) & 0xFFFFu : 0u;
             sh->pool[v.hub0 + w] = lo | (hi << 16);
         }
     } else {
