@@ -29,9 +29,13 @@
 // is not alive holds cnt < lmd, so that decrement is unique).
 //
 // Vertices of each graph are relabelled by total degree, descending, at
-// upload; level 0 gives the heavy head (more than kWaveVertex entries) a
-// wavefront per vertex and the rest a thread per vertex.  Results are
-// written back in the caller's ids at the end of each item.
+// upload.  Level 0 of the items of graphs with at most kPullMaxL items comes
+// from grid-wide pull kernels that run before the peel (k_batch_pull_*); a
+// workgroup pulls the rows of the other graphs' items itself, the heavy
+// head (more than kWaveVertex entries) a wavefront per vertex and the rest
+// a thread per vertex.  The degree rows of an item's heaviest vertices are
+// kept in LDS for the whole chain.  Results are written back in the
+// caller's ids at the end of each item.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -419,7 +423,7 @@ struct PullTile {
     uint32_t g;      // graph
     uint32_t v0;     // first vertex (internal id)
     uint32_t count;  // vertices
-    uint32_t heavy;  // 1: ids < Hp, a wavefront per vertex
+    uint32_t heavy;  // 1: hubs (more than kPullStageBytes entries), a wavefront per vertex
 };
 
 struct PullItems {
@@ -427,8 +431,9 @@ struct PullItems {
     uint64_t sb[hg::kPullMaxL];
 };
 
-// Heavy head (ids < Hp): a wavefront per vertex, lanes strided over the
-// 16-byte words of each layer segment, counts reduced across the wavefront.
+// Hubs (vertices with more than kPullStageBytes entries, the first ids): a
+// wavefront per vertex, lanes strided over the 16-byte words of each layer
+// segment, counts reduced across the wavefront.
 template <uint32_t NJ>
 __global__ void __launch_bounds__(kPullBlock) k_batch_pull_heavy(const BGraph* graphs, const BItem* items_by_id,
                                                                 const PullTile* tiles, uint32_t ntiles,
