@@ -12,8 +12,9 @@
 //   out    int32  [sum L_g*n_g]  core index per graph [lmd][v], caller's ids
 // and per item, in pools: deg [n][L], cnt, idx, list, queue [n], liveness
 // words [ceil(n/32)], and (allocated at the first densest call) the edge
-// histogram [L][kcap].  Internal ids sort each graph's vertices by total
-// degree, descending.
+// histogram [L][kcap].  The items (by id and in queue order) and the level-0
+// pull tiles (graph, first vertex, count) sit in small arrays of their own.
+// Internal ids sort each graph's vertices by total degree, descending.
 
 #include <hip/hip_runtime.h>
 
