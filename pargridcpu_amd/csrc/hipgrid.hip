@@ -109,7 +109,7 @@ void release(hg_graph* g) {
                     g->d_ihist, g->d_insts, g->d_states, g->d_ctl, g->hist, g->d_density, g->d_sel, g->d_cell_len,
                     g->d_pow};
     for (void* p : ptrs)
-        if (p) (void)hipFree(p);
+        if (p) hgc::dfree(p);
     if (g->h_ctl) (void)hipHostFree(g->h_ctl);
     for (auto& e : g->ev)
         if (e) (void)hipEventDestroy(e);
@@ -125,7 +125,7 @@ struct Scratch {
     ~Scratch() {
         if (!ptrs.empty()) (void)hipStreamSynchronize(stream);
         for (void* p : ptrs)
-            if (p) (void)hipFree(p);
+            if (p) hgc::dfree(p);
     }
     template <typename T>
     hipError_t alloc(T** p, size_t count) {

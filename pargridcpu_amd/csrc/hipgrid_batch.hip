@@ -119,13 +119,14 @@ void drop_graph(hg_batch* b) {
 void release(hg_batch* b) {
     if (!b) return;
     (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);  // nothing queued may still use the memory
     drop_graph(b);
     void* ptrs[] = {b->off,     b->adj,     b->adjz,     b->z,        b->deg0,    b->perm,
                     b->out,     b->d_graphs, b->d_items, b->d_items_id, b->d_tiles, b->d_head,
                     b->d_kmax,  b->degpool, b->cntpool,  b->idxpool,  b->listpool, b->qpool,
                     b->bitpool, b->histpool, b->d_sel};
     for (void* p : ptrs)
-        if (p) (void)hipFree(p);
+        if (p) hgc::dfree(p);
     for (auto& e : b->ev)
         if (e) (void)hipEventDestroy(e);
     if (b->own_stream && b->stream) (void)hipStreamDestroy(b->stream);
@@ -420,7 +421,7 @@ int create_impl(const hg_batch_graph* gs, int32_t count, void* stream, hg_batch*
     auto cleanup = [&]() {
         void* t[] = {d_rp, d_col, d_rp_ptr, d_col_ptr, d_off64, d_inv, d_zc, d_prep};
         for (void* p : t)
-            if (p) (void)hipFree(p);
+            if (p) hgc::dfree(p);
     };
     hipError_t e = dalloc(&d_rp, RP);
     if (e == hipSuccess) e = dalloc(&d_col, E);
@@ -698,7 +699,10 @@ int densest_impl(hg_batch* b, const int32_t* graphs, int32_t count, double beta,
     std::vector<BItem> sitems(sel.size());
     if (!sel.empty()) {
         if (sel.size() > b->sel_cap) {
-            if (b->d_sel) (void)hipFree(b->d_sel);
+            if (b->d_sel) {
+                (void)hipStreamSynchronize(b->stream);
+                hgc::dfree(b->d_sel);
+            }
             b->d_sel = nullptr;
             b->sel_cap = 0;
             HIP_TRY(dalloc(&b->d_sel, sel.size()));

@@ -1,7 +1,7 @@
 // hipgrid_common.h -- host-side helpers shared by the engine's translation
 // units: the per-thread error message behind hg_last_error, HIP error
-// mapping, environment knobs, typed device allocation, pinned staging of
-// host<->device copies.
+// mapping, environment knobs, typed device allocation from a cache, pinned
+// staging of host<->device copies.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -10,9 +10,12 @@
 #include <cstddef>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <thread>
+#include <unordered_map>
+#include <utility>
 #include <vector>
 
 #include "hipgrid.h"
@@ -33,11 +36,114 @@ inline int env_int(const char* name, int dflt) {
     return x > 0 ? x : dflt;
 }
 
+// Device memory cache.  Graphs and batches are created and destroyed per
+// request in a serving loop, several lanes at once, and hipMalloc / hipFree
+// are slow and serialise the lanes.  Blocks of up to kPoolMaxBlock bytes are
+// rounded up to a power of two and, when freed, kept per (device, size) for
+// the next allocation, up to HG_POOL_MB of cached memory per process
+// (default 8192; larger blocks and anything beyond the cap are freed at
+// once).  The owner frees only after its stream has drained, so a cached
+// block is idle when it is handed out again; its contents are whatever the
+// last owner left.
+struct DevPool {
+    std::mutex m;
+    std::unordered_map<void*, std::pair<int, size_t>> live;        // block -> (device, size)
+    std::map<std::pair<int, size_t>, std::vector<void*>> idle;     // (device, size) -> blocks
+    size_t cached = 0, cap = 0;
+};
+constexpr size_t kPoolMaxBlock = size_t(512) << 20;
+
+inline DevPool& dev_pool() {
+    static DevPool* pool = [] {
+        DevPool* d = new DevPool();
+        const char* v = std::getenv("HG_POOL_MB");
+        const long mb = v && *v ? std::atol(v) : 8192;
+        d->cap = mb > 0 ? size_t(mb) << 20 : 0;
+        return d;
+    }();
+    return *pool;
+}
+
+// Frees every cached block (when the runtime reports the device full).
+inline void pool_trim() {
+    DevPool& d = dev_pool();
+    std::vector<void*> all;
+    {
+        std::lock_guard<std::mutex> lk(d.m);
+        for (auto& kv : d.idle) {
+            all.insert(all.end(), kv.second.begin(), kv.second.end());
+            kv.second.clear();
+        }
+        d.cached = 0;
+    }
+    for (void* q : all) (void)hipFree(q);
+}
+
+inline hipError_t malloc_or_trim(void** p, size_t bytes) {
+    hipError_t e = hipMalloc(p, bytes);
+    if (e == hipErrorOutOfMemory) {
+        (void)hipGetLastError();
+        pool_trim();
+        e = hipMalloc(p, bytes);
+    }
+    return e;
+}
+
+inline hipError_t pool_malloc(void** p, size_t bytes) {
+    *p = nullptr;
+    if (bytes > kPoolMaxBlock) return malloc_or_trim(p, bytes);
+    size_t c = 256;
+    while (c < bytes) c <<= 1;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    DevPool& d = dev_pool();
+    {
+        std::lock_guard<std::mutex> lk(d.m);
+        auto it = d.idle.find({dev, c});
+        if (it != d.idle.end() && !it->second.empty()) {
+            *p = it->second.back();
+            it->second.pop_back();
+            d.cached -= c;
+            d.live[*p] = {dev, c};
+            return hipSuccess;
+        }
+    }
+    e = malloc_or_trim(p, c);
+    if (e == hipSuccess) {
+        std::lock_guard<std::mutex> lk(d.m);
+        d.live[*p] = {dev, c};
+    }
+    return e;
+}
+
+// Returns a block to the cache (or to the runtime); p must be idle.
+inline void dfree(void* p) {
+    if (!p) return;
+    DevPool& d = dev_pool();
+    {
+        std::lock_guard<std::mutex> lk(d.m);
+        auto it = d.live.find(p);
+        if (it != d.live.end()) {
+            const auto key = it->second;
+            d.live.erase(it);
+            if (d.cached + key.second <= d.cap) {
+                d.idle[key].push_back(p);
+                d.cached += key.second;
+                return;
+            }
+        }
+    }
+    (void)hipFree(p);
+}
+
 template <typename T>
 hipError_t dalloc(T** p, size_t count) {
-    *p = nullptr;
     if (count == 0) count = 1;
-    return hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T));
+    void* q = nullptr;
+    const hipError_t e = pool_malloc(&q, count * sizeof(T));
+    *p = static_cast<T*>(q);
+    return e;
 }
 
 // Runs fn(begin, end) over [0, count) in contiguous ranges of at least
