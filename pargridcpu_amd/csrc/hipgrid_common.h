@@ -38,13 +38,14 @@ inline int env_int(const char* name, int dflt) {
 
 // Device memory cache.  Graphs and batches are created and destroyed per
 // request in a serving loop, several lanes at once, and hipMalloc / hipFree
-// are slow and serialise the lanes.  Blocks of up to kPoolMaxBlock bytes are
-// rounded up to a power of two and, when freed, kept per (device, size) for
-// the next allocation, up to HG_POOL_MB of cached memory per process
-// (default 8192; larger blocks and anything beyond the cap are freed at
-// once).  The owner frees only after its stream has drained, so a cached
-// block is idle when it is handed out again; its contents are whatever the
-// last owner left.
+// are slow and serialise the lanes.  Freed blocks are kept per (device,
+// size) for the next allocation of that size, up to HG_POOL_MB of cached
+// memory per process (default 32768; beyond it blocks are freed at once):
+// blocks of up to kPoolMaxBlock bytes are rounded up to a power of two,
+// larger ones to 2 MB (requests of one shape reuse them).  When the device
+// is full the cache is emptied and the allocation retried.  The owner frees
+// only after its stream has drained, so a cached block is idle when it is
+// handed out again; its contents are whatever the last owner left.
 struct DevPool {
     std::mutex m;
     std::unordered_map<void*, std::pair<int, size_t>> live;        // block -> (device, size)
@@ -57,7 +58,7 @@ inline DevPool& dev_pool() {
     static DevPool* pool = [] {
         DevPool* d = new DevPool();
         const char* v = std::getenv("HG_POOL_MB");
-        const long mb = v && *v ? std::atol(v) : 8192;
+        const long mb = v && *v ? std::atol(v) : 32768;
         d->cap = mb > 0 ? size_t(mb) << 20 : 0;
         return d;
     }();
@@ -91,9 +92,13 @@ inline hipError_t malloc_or_trim(void** p, size_t bytes) {
 
 inline hipError_t pool_malloc(void** p, size_t bytes) {
     *p = nullptr;
-    if (bytes > kPoolMaxBlock) return malloc_or_trim(p, bytes);
     size_t c = 256;
-    while (c < bytes) c <<= 1;
+    if (bytes > kPoolMaxBlock) {
+        const size_t g = size_t(2) << 20;
+        c = (bytes + g - 1) / g * g;
+    } else {
+        while (c < bytes) c <<= 1;
+    }
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess) return e;
