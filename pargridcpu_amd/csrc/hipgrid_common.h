@@ -10,6 +10,7 @@
 #include <cstddef>
 #include <cstdlib>
 #include <cstring>
+#include <list>
 #include <map>
 #include <mutex>
 #include <string>
@@ -40,16 +41,18 @@ inline int env_int(const char* name, int dflt) {
 // request in a serving loop, several lanes at once, and hipMalloc / hipFree
 // are slow and serialise the lanes.  Freed blocks are kept per (device,
 // size) for the next allocation of that size, up to HG_POOL_MB of cached
-// memory per process (default 32768; beyond it blocks are freed at once):
+// memory per process (default 32768; the blocks freed longest ago go first):
 // blocks of up to kPoolMaxBlock bytes are rounded up to a power of two,
 // larger ones to 2 MB (requests of one shape reuse them).  When the device
 // is full the cache is emptied and the allocation retried.  The owner frees
 // only after its stream has drained, so a cached block is idle when it is
 // handed out again; its contents are whatever the last owner left.
 struct DevPool {
+    using Key = std::pair<int, size_t>;  // (device, block bytes)
     std::mutex m;
-    std::unordered_map<void*, std::pair<int, size_t>> live;        // block -> (device, size)
-    std::map<std::pair<int, size_t>, std::vector<void*>> idle;     // (device, size) -> blocks
+    std::unordered_map<void*, Key> live;
+    std::list<std::pair<Key, void*>> idle;  // least recently freed first
+    std::map<Key, std::vector<std::list<std::pair<Key, void*>>::iterator>> by_key;
     size_t cached = 0, cap = 0;
 };
 constexpr size_t kPoolMaxBlock = size_t(512) << 20;
@@ -71,10 +74,9 @@ inline void pool_trim() {
     std::vector<void*> all;
     {
         std::lock_guard<std::mutex> lk(d.m);
-        for (auto& kv : d.idle) {
-            all.insert(all.end(), kv.second.begin(), kv.second.end());
-            kv.second.clear();
-        }
+        for (auto& kv : d.idle) all.push_back(kv.second);
+        d.idle.clear();
+        d.by_key.clear();
         d.cached = 0;
     }
     for (void* q : all) (void)hipFree(q);
@@ -92,6 +94,8 @@ inline hipError_t malloc_or_trim(void** p, size_t bytes) {
 
 inline hipError_t pool_malloc(void** p, size_t bytes) {
     *p = nullptr;
+    DevPool& d = dev_pool();
+    if (d.cap == 0) return malloc_or_trim(p, bytes);  // cache off: exact sizes, freed at once
     size_t c = 256;
     if (bytes > kPoolMaxBlock) {
         const size_t g = size_t(2) << 20;
@@ -102,44 +106,59 @@ inline hipError_t pool_malloc(void** p, size_t bytes) {
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess) return e;
-    DevPool& d = dev_pool();
+    const DevPool::Key key{dev, c};
     {
         std::lock_guard<std::mutex> lk(d.m);
-        auto it = d.idle.find({dev, c});
-        if (it != d.idle.end() && !it->second.empty()) {
-            *p = it->second.back();
+        auto it = d.by_key.find(key);
+        if (it != d.by_key.end() && !it->second.empty()) {
+            auto li = it->second.back();
             it->second.pop_back();
+            *p = li->second;
+            d.idle.erase(li);
             d.cached -= c;
-            d.live[*p] = {dev, c};
+            d.live[*p] = key;
             return hipSuccess;
         }
     }
     e = malloc_or_trim(p, c);
     if (e == hipSuccess) {
         std::lock_guard<std::mutex> lk(d.m);
-        d.live[*p] = {dev, c};
+        d.live[*p] = key;
     }
     return e;
 }
 
-// Returns a block to the cache (or to the runtime); p must be idle.
+// Returns a block to the cache (or to the runtime); p must be idle.  The
+// blocks freed longest ago make room for it when the cache is full.
 inline void dfree(void* p) {
     if (!p) return;
     DevPool& d = dev_pool();
+    std::vector<void*> evict;
+    bool kept = false;
     {
         std::lock_guard<std::mutex> lk(d.m);
         auto it = d.live.find(p);
         if (it != d.live.end()) {
-            const auto key = it->second;
+            const DevPool::Key key = it->second;
             d.live.erase(it);
-            if (d.cached + key.second <= d.cap) {
-                d.idle[key].push_back(p);
+            if (key.second <= d.cap) {
+                while (d.cached + key.second > d.cap && !d.idle.empty()) {
+                    auto old = d.idle.begin();
+                    auto& v = d.by_key[old->first];
+                    v.erase(std::find(v.begin(), v.end(), old));
+                    d.cached -= old->first.second;
+                    evict.push_back(old->second);
+                    d.idle.erase(old);
+                }
+                d.idle.emplace_back(key, p);
+                d.by_key[key].push_back(std::prev(d.idle.end()));
                 d.cached += key.second;
-                return;
+                kept = true;
             }
         }
     }
-    (void)hipFree(p);
+    for (void* q : evict) (void)hipFree(q);
+    if (!kept) (void)hipFree(p);
 }
 
 template <typename T>
