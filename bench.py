@@ -168,8 +168,9 @@ def run_serving(n_requests: int = 160, rates=(20.0, 200.0)) -> dict:
     """Serving form: Poisson arrivals, 80% decompositions and 20%
     densest-subgraph searches; graphs from the batched workload's
     distribution, 10% of them large (n = 5M, L = 12, 60M sampled edges);
-    200 GB device-memory budget.  The request loop (4 lanes: small requests
-    batched when 8 or more wait, else run alone on the single-graph engine)
+    200 GB device-memory budget.  The request loop (4 lanes: requests of at
+    most 4M directed entries batched when 8 or more wait, all others run
+    alone on the single-graph engine)
     against one-at-a-time execution on the single-graph engine, same arrival
     trace, at each offered rate (20 req/s: the GPU is not saturated; 200
     req/s: arrivals outpace one-at-a-time service and the queue builds up).
@@ -217,7 +218,7 @@ def run_serving(n_requests: int = 160, rates=(20.0, 200.0)) -> dict:
         trace = make_trace(rate)
         res = {"offered_rate": rate}
         outs0 = None
-        for name, kw in (("continuous", dict(lanes=4, max_batch=64, large_entries=20_000_000)),
+        for name, kw in (("continuous", dict(lanes=4, max_batch=64, large_entries=4_000_000)),
                          ("one_at_a_time", dict(plain=True))):
             st, infos, outs = serve(trace, **kw)
             sizes = [i.batch_size for i in infos]

@@ -426,7 +426,7 @@ class Server:
     decomposition or densest-subgraph search, wait for their answers."""
 
     def __init__(self, hbm_budget: int = 200 << 30, lanes: int = 4, max_batch: int = 64,
-                 large_entries: int = 50_000_000, plain: bool = False, params: dict | None = None,
+                 large_entries: int = 4_000_000, plain: bool = False, params: dict | None = None,
                  min_batch: int = 0):
         cfg = ServerConfig()
         cfg.hbm_budget = int(hbm_budget)
