@@ -362,6 +362,24 @@ __device__ void peel_cascade(const ItemView& v, PeelShared<B>* sh, int32_t k) {
     __syncthreads();
 }
 
+// #{l < L : row[l] >= thr} for a degree row in global memory, two words per
+// device-scope load where the row is 8-byte aligned (half the requests of a
+// word at a time; each 64-bit load sees two words that atomics update).
+__device__ __forceinline__ uint32_t row_count(const uint32_t* row, uint32_t L, uint32_t thr) {
+    uint32_t c = 0, l = 0;
+    if ((reinterpret_cast<uintptr_t>(row) & 7u) != 0 && L > 0) {
+        c += ld_agent(&row[0]) >= thr;
+        l = 1;
+    }
+    for (; l + 2 <= L; l += 2) {
+        const unsigned long long w = __hip_atomic_load(reinterpret_cast<const unsigned long long*>(row + l),
+                                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        c += uint32_t(uint32_t(w) >= thr) + uint32_t(uint32_t(w >> 32) >= thr);
+    }
+    if (l < L) c += ld_agent(&row[l]) >= thr;
+    return c;
+}
+
 // Scan of level k >= 1 over the alive list, compacted in place.
 template <int B>
 __device__ void peel_scan(const ItemView& v, PeelShared<B>* sh, int32_t k) {
@@ -382,7 +400,7 @@ __device__ void peel_scan(const ItemView& v, PeelShared<B>* sh, int32_t k) {
             if (u < v.T)
                 for (uint32_t l = 0; l < L; ++l) c += hub_get<B>(v, sh, u * L + l) >= thr;
             else
-                for (uint32_t l = 0; l < L; ++l) c += ld_agent(&v.deg[uint64_t(u) * L + l]) >= thr;
+                c = row_count(v.deg + uint64_t(u) * L, L, thr);
         }
         const bool rem = live && c < lmd;
         const bool keep = live && !rem;
