@@ -741,6 +741,23 @@ k_init(const Inst* insts, InstState* states, Control* ctl, uint32_t I,
     }
 }
 
+// #{l < L : d[l] >= thr} for a degree row written by earlier kernels; rows
+// of a multiple of four words are 16-byte aligned (the rows start at v * L
+// words of a 256-byte aligned array) and read four words per load.
+__device__ __forceinline__ uint32_t row_count_ge(const uint32_t* d, uint32_t L, uint32_t thr) {
+    uint32_t c = 0;
+    if ((L & 3u) == 0) {
+        const uint4* q = reinterpret_cast<const uint4*>(d);
+        for (uint32_t w = 0; w < L / 4; ++w) {
+            const uint4 x = q[w];
+            c += uint32_t(x.x >= thr) + uint32_t(x.y >= thr) + uint32_t(x.z >= thr) + uint32_t(x.w >= thr);
+        }
+    } else {
+        for (uint32_t l = 0; l < L; ++l) c += d[l] >= thr;
+    }
+    return c;
+}
+
 // Level k >= 1 scan of every instance in phase Scan: walk the block's alive
 // list, drop vertices removed since (idx >= 0), remove those with fewer than
 // lmd layers of degree >= k+1, keep the rest with their cnt.  The list is
@@ -774,10 +791,7 @@ __global__ void __launch_bounds__(kBlock) k_scan(const Inst* insts, InstState* s
                 live = is_alive(in, v);
             }
             uint32_t c = 0;
-            if (live) {
-                const uint32_t* d = rec_of(in, v);
-                for (uint32_t l = 0; l < L; ++l) c += d[l] >= thr;
-            }
+            if (live) c = row_count_ge(rec_of(in, v), L, thr);
             const bool rem = live && c < in.lmd;
             const bool keep = live && !rem;
             if (rem) {
