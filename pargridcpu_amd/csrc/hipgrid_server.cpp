@@ -297,6 +297,10 @@ int hg_server_create(const hg_server_config* cfg, hg_server** out) {
     if (!cfg->plain && cfg->max_batch < 1) return fail(HG_ERR_ARG, "max_batch must be >= 1");
     std::unique_ptr<hg_server> s(new (std::nothrow) hg_server());
     if (!s) return fail(HG_ERR_NOMEM, "host allocation failed");
+    // Start from an empty device-memory cache: blocks of earlier, differently
+    // sized work would otherwise only be evicted (hipFree, which stalls every
+    // lane) one by one while requests are served.
+    hgc::pool_trim();
     s->cfg = *cfg;
     if (s->cfg.min_batch < 0) return fail(HG_ERR_ARG, "min_batch must be >= 0");
     if (s->cfg.min_batch == 0) s->cfg.min_batch = 8;
