@@ -134,3 +134,33 @@ def test_statistics_line():
     s = g.PrintStatistics()
     m = [lay.getM() >> 1 for lay in g._layers]
     assert s == f"|L| = 3, |V| = 300, |E| = {sum(m)}, max_|E_i| = {max(m)}"
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU; asked for more it says so and exits."""
+    import subprocess
+    import sys
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode != 0
+    assert "one GPU" in r.stdout + r.stderr
+
+
+def test_committed_profiles_carry_traffic():
+    """The bench line's roofline traffic comes from the committed rocprofv3
+    summaries; each holds its figure and the commands that produced it."""
+    import importlib.util
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    spec = importlib.util.spec_from_file_location("bench_mod", os.path.join(root, "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    for name, key in (("hg_decompose_c3", "traffic_bytes_per_decomposition"),
+                      ("hg_batch_peel_c4", "traffic_bytes_per_launch")):
+        val, src = bench.measured_traffic(name, key)
+        assert src == os.path.join("profiles", f"{name}.md")
+        assert val is not None and val > 10**10
+        text = open(os.path.join(root, src)).read()
+        assert "rocprofv3 --kernel-trace --stats" in text and "--pmc FETCH_SIZE" in text and "--pmc WRITE_SIZE" in text
