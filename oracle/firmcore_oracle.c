@@ -23,16 +23,24 @@
  *     against each other, against orc_bruteforce on small graphs, and against
  *     closed forms (L = 1 k-core numbers, cliques, stars, cycles, paths,
  *     identical layers).
+ *   orc_coreindex_lmd, orc_coreindex_merged_lmd, orc_merge: orc_coreindex is
+ *     orc_merge followed by the loop of orc_coreindex_merged_lmd over lmd,
+ *     so every pin of orc_coreindex covers them; the golden tests also run
+ *     them from threads, one lmd each.
+ *   orc_check_lmd: accepts every golden core index written by the reference,
+ *     and rejects each planted off-by-one (+1 or -1 at one vertex) of them.
  *   orc_bruteforce: definition-level fixed point, checked by the closed forms.
  *   orc_wds: against the reference's printed WdsList result (golden), plus a
  *     brute-force per-layer induced edge count on small graphs.
  *   orc_wds_coreindex: against get_wds's per-lmd results from the reference
  *     built from source (golden), plus an independent numpy recomputation.
  */
+#define _DEFAULT_SOURCE
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#include <sys/mman.h>
 
 typedef struct {
     int64_t n;
@@ -422,10 +430,20 @@ int orc_wds(int64_t n, int32_t L, const int64_t* const* row_ptr, const uint32_t*
 
 /* --------------------------------------------------------------- CoreIndex
  * Restates get_index2 + findLmdthLargest (CoreIndexParallel/CoreIndex.cpp:
- * 46-109) per lmd, as CoreIndex::Execute (:168-213) runs it: bucket peeling
- * on the lmd-th largest layer degree.  std::set buckets become bucket
- * stacks with lazy deletion (a vertex's live bucket is bucket_of[v]);
- * the index does not depend on the pop order within a bucket. */
+ * 46-109) for one lmd.  CoreIndex::Execute (:168-213) runs it for lmd =
+ * 1..L one after another on fresh degree copies; CoreIndex::ExecuteParallel
+ * (:217-300, "ParCoreIndex") runs the lmds as parallel loop iterations, each
+ * on its own copy -- orc_coreindex_lmd is one such iteration and may be
+ * called from several threads at once (it shares nothing but the read-only
+ * graph).  Bucket peeling on the lmd-th largest layer degree; std::set
+ * buckets become bucket stacks with lazy deletion (a vertex's live bucket is
+ * bucket_of[v]); the index does not depend on the pop order within a bucket.
+ * The neighbour loop decrements every neighbour's layer degree and masks the
+ * step for neighbours already at or below k (the reference skips them with
+ * `continue`): their degrees are never read again, so the result is the
+ * same, and the loop has no data-dependent branch on a cache miss.  Large
+ * arrays are requested in transparent huge pages (a 100M-vertex graph's
+ * degree array is 3.2 GB of random accesses). */
 static uint32_t lmd_th_largest(const uint32_t* deg, uint32_t lmd, int L, uint32_t* tmp) {
     memcpy(tmp, deg, sizeof(uint32_t) * (size_t)L);
     /* partial selection sort, descending, first lmd entries */
@@ -457,68 +475,195 @@ static int ustack_push(ustack* s, uint32_t x) {
     return 0;
 }
 
-int orc_coreindex(int64_t n, int32_t L, const int64_t* const* row_ptr, const uint32_t* const* col,
-                  int32_t* core_index) {
-    if (n <= 0 || L <= 0 || L > 64) return -1;
-    orc_graph g = {n, L, row_ptr, col};
-    uint32_t* degs = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(n * L));
-    uint32_t* iv = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)n);
-    int64_t* bucket_of = (int64_t*)malloc(sizeof(int64_t) * (size_t)n);
-    uint8_t* in_n = (uint8_t*)calloc((size_t)n, 1);
-    uint32_t* nlist = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)n);
-    ustack* B = (ustack*)calloc((size_t)n + 1, sizeof(ustack));
-    uint32_t tmp[64];
-    int rc = 0;
-    if (!degs || !iv || !bucket_of || !in_n || !nlist || !B) { rc = -3; goto done; }
-    for (uint32_t lmd = 1; lmd <= (uint32_t)L; ++lmd) {
-        for (int64_t v = 0; v < n; ++v)
-            for (int l = 0; l < L; ++l) degs[v * L + l] = (uint32_t)deg_of(&g, l, v);
-        for (int64_t b = 0; b <= n; ++b) B[b].size = 0;
-        for (int64_t v = 0; v < n; ++v) {
-            iv[v] = lmd_th_largest(degs + v * L, lmd, L, tmp);
-            bucket_of[v] = iv[v];
-            if (ustack_push(&B[iv[v]], (uint32_t)v)) { rc = -3; goto done; }
+static void* big_alloc(size_t bytes) {
+    if (bytes == 0) bytes = 1;
+    void* p = mmap(NULL, bytes, PROT_READ | PROT_WRITE, MAP_PRIVATE | MAP_ANONYMOUS, -1, 0);
+    if (p == MAP_FAILED) return NULL;
+#ifdef MADV_HUGEPAGE
+    madvise(p, bytes, MADV_HUGEPAGE);
+#endif
+    return p;
+}
+
+static void big_free(void* p, size_t bytes) {
+    if (p) munmap(p, bytes ? bytes : 1);
+}
+
+/* Vertex-major merged adjacency for the CoreIndex loop: the entries of v in
+ * every layer are adj[off[v] .. off[v+1]), layer 0 first, each entry stored
+ * as u * L + l -- the index of degs[u][l] in a row-major [n][L] array.  A
+ * popped vertex's neighbourhood is then one contiguous run instead of L
+ * separate rows (the reference walks adj_lst[vv] of every layer,
+ * CoreIndex.cpp:84-95; same entries, same order within a layer).  Requires
+ * n * L < 2^32.  off has n+1 entries, adj sum of all row lengths. */
+int orc_merge(int64_t n, int32_t L, const int64_t* const* row_ptr, const uint32_t* const* col, int64_t* off,
+              uint32_t* adj) {
+    if (n <= 0 || L <= 0 || L > 64 || (uint64_t)n * (uint64_t)L >= (1ull << 32)) return -1;
+    int64_t e = 0;
+    for (int64_t v = 0; v < n; ++v) {
+        off[v] = e;
+        for (int l = 0; l < L; ++l) {
+            const uint32_t* nb = col[l] + row_ptr[l][v];
+            const int64_t d = row_ptr[l][v + 1] - row_ptr[l][v];
+            for (int64_t t = 0; t < d; ++t) adj[e++] = nb[t] * (uint32_t)L + (uint32_t)l;
         }
-        for (int64_t k = 0; k <= n; ++k) {
-            while (B[k].size > 0) {
-                const uint32_t vv = B[k].v[--B[k].size];
-                if (bucket_of[vv] != k) continue; /* stale entry */
-                bucket_of[vv] = -1;               /* popped */
-                iv[vv] = (uint32_t)k;
-                int64_t nn = 0;
-                for (int l = 0; l < L; ++l) {
-                    const uint32_t* nb = g.col[l] + g.row_ptr[l][vv];
-                    const int64_t d = deg_of(&g, l, vv);
-                    for (int64_t t = 0; t < d; ++t) {
-                        const uint32_t u = nb[t];
-                        if (iv[u] <= (uint32_t)k) continue;
-                        degs[(int64_t)u * L + l] -= 1;
-                        if (degs[(int64_t)u * L + l] + 1 == iv[u] && !in_n[u]) {
-                            in_n[u] = 1;
-                            nlist[nn++] = u;
-                        }
-                    }
-                }
-                for (int64_t t = 0; t < nn; ++t) {
-                    const uint32_t u = nlist[t];
-                    in_n[u] = 0;
-                    iv[u] = lmd_th_largest(degs + (int64_t)u * L, lmd, L, tmp);
-                    const int64_t nb = iv[u] > (uint32_t)k ? (int64_t)iv[u] : k;
-                    bucket_of[u] = nb;
-                    if (ustack_push(&B[nb], u)) { rc = -3; goto done; }
+    }
+    off[n] = e;
+    return 0;
+}
+
+/* get_index2 for one lmd over the merged adjacency (see above). */
+int orc_coreindex_merged_lmd(int64_t n, int32_t L, const int64_t* off, const uint32_t* adj, int32_t lmd,
+                             int32_t* core_row) {
+    if (n <= 0 || L <= 0 || L > 64 || lmd < 1 || lmd > L) return -1;
+    const size_t degs_b = sizeof(uint32_t) * (size_t)n * (size_t)L;
+    uint32_t* degs = (uint32_t*)big_alloc(degs_b);
+    uint32_t* iv = (uint32_t*)big_alloc(sizeof(uint32_t) * (size_t)n);
+    int32_t* bucket_of = (int32_t*)big_alloc(sizeof(int32_t) * (size_t)n);
+    uint8_t* in_n = (uint8_t*)big_alloc((size_t)n);
+    uint32_t* nlist = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)n);
+    ustack* B = NULL;
+    uint32_t tmp[64];
+    uint32_t kcap = 0;
+    const uint32_t uL = (uint32_t)L;
+    int rc = 0;
+    if (!degs || !iv || !bucket_of || !in_n || !nlist) { rc = -3; goto done; }
+    /* the degree copy of CoreIndex.cpp:181-191 / :254-258 */
+    memset(degs, 0, degs_b);
+    for (int64_t v = 0; v < n; ++v)
+        for (int64_t t = off[v]; t < off[v + 1]; ++t) degs[v * L + (adj[t] % uL)] += 1;
+    /* CoreIndex.cpp:71-74 */
+    for (int64_t v = 0; v < n; ++v) {
+        iv[v] = lmd_th_largest(degs + v * L, (uint32_t)lmd, L, tmp);
+        if (iv[v] > kcap) kcap = iv[v];
+    }
+    B = (ustack*)calloc((size_t)kcap + 1, sizeof(ustack));
+    if (!B) { rc = -3; goto done; }
+    for (int64_t v = 0; v < n; ++v) {
+        bucket_of[v] = (int32_t)iv[v];
+        if (ustack_push(&B[iv[v]], (uint32_t)v)) { rc = -3; goto done; }
+    }
+    /* CoreIndex.cpp:77-104; buckets above kcap stay empty */
+    for (int64_t k = 0; k <= (int64_t)kcap; ++k) {
+        while (B[k].size > 0) {
+            const uint32_t vv = B[k].v[--B[k].size];
+            if (bucket_of[vv] != k) continue; /* stale entry */
+            bucket_of[vv] = -1;               /* popped */
+            iv[vv] = (uint32_t)k;
+            int64_t nn = 0;
+            for (int64_t t = off[vv]; t < off[vv + 1]; ++t) {
+                const uint32_t e = adj[t];
+                const uint32_t u = e / uL;
+                const uint32_t ivu = iv[u];
+                const uint32_t live = ivu > (uint32_t)k; /* i_v[nb_u] <= k: continue */
+                const uint32_t nd = degs[e] - live;
+                degs[e] = nd;
+                if ((live & (nd + 1 == ivu)) && !in_n[u]) {
+                    in_n[u] = 1;
+                    nlist[nn++] = u;
                 }
             }
+            for (int64_t t = 0; t < nn; ++t) {
+                const uint32_t u = nlist[t];
+                in_n[u] = 0;
+                iv[u] = lmd_th_largest(degs + (int64_t)u * L, (uint32_t)lmd, L, tmp);
+                const int64_t nb = iv[u] > (uint32_t)k ? (int64_t)iv[u] : k;
+                bucket_of[u] = (int32_t)nb;
+                if (ustack_push(&B[nb], u)) { rc = -3; goto done; }
+            }
         }
-        int32_t* row = core_index + (size_t)(lmd - 1) * (size_t)n;
-        for (int64_t v = 0; v < n; ++v) row[v] = (int32_t)iv[v];
     }
+    for (int64_t v = 0; v < n; ++v) core_row[v] = (int32_t)iv[v];
 done:
     if (B) {
-        for (int64_t b = 0; b <= n; ++b) free(B[b].v);
+        for (int64_t b = 0; b <= (int64_t)kcap; ++b) free(B[b].v);
         free(B);
     }
-    free(degs); free(iv); free(bucket_of); free(in_n); free(nlist);
+    big_free(degs, degs_b);
+    big_free(iv, sizeof(uint32_t) * (size_t)n);
+    big_free(bucket_of, sizeof(int32_t) * (size_t)n);
+    big_free(in_n, (size_t)n);
+    free(nlist);
     return rc;
+}
+
+static int64_t total_entries(int64_t n, int32_t L, const int64_t* const* row_ptr) {
+    int64_t m = 0;
+    for (int l = 0; l < L; ++l) m += row_ptr[l][n] - row_ptr[l][0];
+    return m;
+}
+
+/* CoreIndex::Execute for the listed lmds (lmds == NULL: 1..L); the row of
+ * lmds[i] goes to core_index + i * n. */
+static int coreindex_lmds(int64_t n, int32_t L, const int64_t* const* row_ptr, const uint32_t* const* col,
+                          const int32_t* lmds, int32_t count, int32_t* core_index) {
+    if (n <= 0 || L <= 0 || L > 64 || (uint64_t)n * (uint64_t)L >= (1ull << 32)) return -1;
+    const int64_t m = total_entries(n, L, row_ptr);
+    int64_t* off = (int64_t*)big_alloc(sizeof(int64_t) * (size_t)(n + 1));
+    uint32_t* adj = (uint32_t*)big_alloc(sizeof(uint32_t) * (size_t)m);
+    int rc = (!off || !adj) ? -3 : orc_merge(n, L, row_ptr, col, off, adj);
+    for (int32_t i = 0; rc == 0 && i < count; ++i) {
+        const int32_t lmd = lmds ? lmds[i] : i + 1;
+        rc = orc_coreindex_merged_lmd(n, L, off, adj, lmd, core_index + (size_t)i * (size_t)n);
+    }
+    big_free(off, sizeof(int64_t) * (size_t)(n + 1));
+    big_free(adj, sizeof(uint32_t) * (size_t)m);
+    return rc;
+}
+
+/* One lmd of CoreIndex (one iteration of ParCoreIndex's loop) into core_row. */
+int orc_coreindex_lmd(int64_t n, int32_t L, const int64_t* const* row_ptr, const uint32_t* const* col, int32_t lmd,
+                      int32_t* core_row) {
+    if (lmd < 1 || lmd > L) return -1;
+    return coreindex_lmds(n, L, row_ptr, col, &lmd, 1, core_row);
+}
+
+/* CoreIndex::Execute (CoreIndex.cpp:168-213): every lmd in turn. */
+int orc_coreindex(int64_t n, int32_t L, const int64_t* const* row_ptr, const uint32_t* const* col,
+                  int32_t* core_index) {
+    return coreindex_lmds(n, L, row_ptr, col, NULL, L, core_index);
+}
+
+/* ------------------------------------------------------ local FirmCore check
+ * Two local conditions every FirmCore index row c (one lmd) satisfies,
+ * independent of any peeling routine; counts the vertices of [v0, v1) that
+ * violate one of them:
+ *   (valid) every v with c(v) = k >= 1 has at least lmd layers l with
+ *           |{u in N_l(v) : c(u) >= k}| >= k -- S(k, lmd) = {c >= k} meets
+ *           the FirmCore condition at v;
+ *   (tight) every v with c(v) = k has fewer than lmd layers l with
+ *           |{u in N_l(v) : c(u) >= k+1}| >= k+1 -- else S(k+1, lmd) plus v
+ *           would be a larger (k+1, lmd) set.
+ * (valid) alone is met by a row that is too low everywhere and (tight) alone
+ * by one that is too high, so together they reject an off-by-one in either
+ * direction at any vertex.  first_bad (may be NULL) gets the smallest
+ * violating id, or -1.  Read-only; threads may split [0, n). */
+int64_t orc_check_lmd(int64_t n, int32_t L, const int64_t* const* row_ptr, const uint32_t* const* col, int32_t lmd,
+                      const int32_t* c, int64_t v0, int64_t v1, int64_t* first_bad) {
+    if (first_bad) *first_bad = -1;
+    if (n <= 0 || L <= 0 || lmd < 1 || lmd > L || v0 < 0 || v1 > n) return -1;
+    int64_t bad = 0;
+    for (int64_t v = v0; v < v1; ++v) {
+        const int32_t k = c[v];
+        int good = 0, over = 0;
+        for (int l = 0; l < L; ++l) {
+            const uint32_t* nb = col[l] + row_ptr[l][v];
+            const int64_t d = row_ptr[l][v + 1] - row_ptr[l][v];
+            int64_t ge = 0, gt = 0;
+            for (int64_t t = 0; t < d; ++t) {
+                const int32_t cu = c[nb[t]];
+                ge += cu >= k;
+                gt += cu > k;
+            }
+            good += ge >= k;
+            over += gt >= (int64_t)k + 1;
+        }
+        if ((k >= 1 && good < lmd) || over >= lmd || k < 0) {
+            if (first_bad && *first_bad < 0) *first_bad = v;
+            ++bad;
+        }
+    }
+    return bad;
 }
 
 /* ------------------------------------------------------------ WdsCoreIndex

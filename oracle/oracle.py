@@ -23,6 +23,7 @@ def lib():
         if not os.path.exists(_LIB_PATH):
             raise RuntimeError("oracle/liboracle.so is not built; run __graft_entry__.build()")
         _lib = C.CDLL(_LIB_PATH)
+        _lib.orc_check_lmd.restype = C.c_int64
     return _lib
 
 
@@ -86,6 +87,94 @@ def coreindex(n, layers):
     if rc != 0:
         raise RuntimeError(f"orc_coreindex failed {rc}")
     return ci
+
+
+def coreindex_lmd(n, layers, lmd, out=None):
+    """One lmd of the CoreIndex restatement (one iteration of ParCoreIndex's
+    parallel loop, CoreIndex.cpp:217-300) into out (int32[n])."""
+    L = len(layers)
+    RP, CL, keep = _layer_ptrs(layers)
+    if out is None:
+        out = np.zeros(n, dtype=np.int32)
+    rc = lib().orc_coreindex_lmd(C.c_int64(n), C.c_int32(L), RP, CL, C.c_int32(lmd), _i32p(out))
+    if rc != 0:
+        raise RuntimeError(f"orc_coreindex_lmd failed {rc}")
+    return out
+
+
+def merged(n, layers):
+    """Vertex-major merged adjacency (orc_merge): off int64[n+1], adj
+    uint32[D] with entries u * L + l."""
+    L = len(layers)
+    RP, CL, keep = _layer_ptrs(layers)
+    D = int(sum(int(l.row_ptr[-1] - l.row_ptr[0]) for l in layers))
+    off = np.empty(n + 1, dtype=np.int64)
+    adj = np.empty(max(D, 1), dtype=np.uint32)
+    rc = lib().orc_merge(C.c_int64(n), C.c_int32(L), RP, CL, _i64p(off), adj.ctypes.data_as(C.POINTER(C.c_uint32)))
+    if rc != 0:
+        raise RuntimeError(f"orc_merge failed {rc}")
+    return off, adj
+
+
+def coreindex_parallel(n, layers, threads=None, lmds=None, progress=None):
+    """CoreIndex restatement with the lmds run concurrently, one host thread
+    each (as ParCoreIndex runs them, CoreIndex.cpp:252) over one shared
+    merged adjacency; ctypes releases the GIL during the calls.  Returns
+    core_index[L][n] (rows not in `lmds` stay 0).  progress(lmd, seconds)
+    is called as each lmd finishes."""
+    import concurrent.futures as cf
+    import time
+
+    L = len(layers)
+    off, adj = merged(n, layers)
+    ci = np.zeros((L, n), dtype=np.int32)
+    todo = list(range(1, L + 1)) if lmds is None else list(lmds)
+    threads = threads or min(len(todo), os.cpu_count() or 1, 16)
+    t0 = time.perf_counter()
+
+    def one(lmd):
+        rc = lib().orc_coreindex_merged_lmd(C.c_int64(n), C.c_int32(L), _i64p(off),
+                                            adj.ctypes.data_as(C.POINTER(C.c_uint32)), C.c_int32(lmd),
+                                            _i32p(ci[lmd - 1]))
+        if rc != 0:
+            raise RuntimeError(f"orc_coreindex_merged_lmd(lmd={lmd}) failed {rc}")
+        if progress:
+            progress(lmd, time.perf_counter() - t0)
+
+    with cf.ThreadPoolExecutor(max_workers=max(1, threads)) as ex:
+        list(ex.map(one, todo))
+    return ci
+
+
+def check_local(n, layers, core_index, threads=None, lmds=None):
+    """Local FirmCore conditions (orc_check_lmd) on every row of core_index;
+    returns {lmd: (violations, first violating id)} for the rows checked."""
+    import concurrent.futures as cf
+
+    L = len(layers)
+    RP, CL, keep = _layer_ptrs(layers)
+    ci = np.ascontiguousarray(core_index, dtype=np.int32)
+    todo = list(range(1, L + 1)) if lmds is None else list(lmds)
+    threads = threads or min(16, os.cpu_count() or 1)
+    parts = max(1, threads)
+    bounds = [n * i // parts for i in range(parts + 1)]
+    jobs = [(lmd, bounds[i], bounds[i + 1]) for lmd in todo for i in range(parts) if bounds[i + 1] > bounds[i]]
+
+    def one(job):
+        lmd, v0, v1 = job
+        fb = C.c_int64(-1)
+        bad = lib().orc_check_lmd(C.c_int64(n), C.c_int32(L), RP, CL, C.c_int32(lmd), _i32p(ci[lmd - 1]),
+                                  C.c_int64(v0), C.c_int64(v1), C.byref(fb))
+        if bad < 0:
+            raise RuntimeError("orc_check_lmd: bad arguments")
+        return lmd, bad, fb.value
+
+    out = {lmd: (0, -1) for lmd in todo}
+    with cf.ThreadPoolExecutor(max_workers=parts) as ex:
+        for lmd, bad, fb in ex.map(one, jobs):
+            b0, f0 = out[lmd]
+            out[lmd] = (b0 + bad, f0 if f0 >= 0 else fb)
+    return out
 
 
 def bruteforce(n, layers):

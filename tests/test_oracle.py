@@ -243,3 +243,39 @@ def test_planted_mistake_is_caught():
 def test_fixture_set_complete():
     names = {os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "*.npz"))}
     assert set(GRAPHS) | {"loader"} <= names
+
+
+@pytest.mark.parametrize("name", sorted(GRAPHS))
+def test_coreindex_parallel_matches_reference(name):
+    """The per-lmd entry run from concurrent threads (ParCoreIndex's shape)
+    gives the reference's golden index."""
+    g, fx = fixture_graph(name)
+    assert np.array_equal(O.coreindex_parallel(g.n, g.layers(), threads=4), fx["core_index"])
+    lmd = g.L
+    assert np.array_equal(O.coreindex_lmd(g.n, g.layers(), lmd), fx["core_index"][lmd - 1])
+
+
+@pytest.mark.parametrize("name", sorted(GRAPHS))
+def test_local_check_accepts_reference_and_rejects_off_by_one(name):
+    """orc_check_lmd passes the reference's own index and flags a planted
+    +1 / -1 at single vertices (the mistake is planted in a copy of the
+    golden index, on the CPU)."""
+    g, fx = fixture_graph(name)
+    ref = fx["core_index"]
+    res = O.check_local(g.n, g.layers(), ref, threads=3)
+    assert all(bad == 0 for bad, _ in res.values()), res
+    rng = np.random.default_rng(7)
+    for lmd in sorted({1, (g.L + 1) // 2, g.L}):
+        row = ref[lmd - 1]
+        cands = np.flatnonzero(row > 0)
+        for v in rng.choice(cands, size=min(4, cands.size), replace=False):
+            for delta in (+1, -1):
+                bad = ref.copy()
+                bad[lmd - 1, v] += delta
+                r = O.check_local(g.n, g.layers(), bad, threads=2, lmds=[lmd])[lmd]
+                assert r[0] > 0, (lmd, int(v), delta)
+        zeros = np.flatnonzero(row == 0)
+        if zeros.size:
+            bad = ref.copy()
+            bad[lmd - 1, zeros[0]] = 1
+            assert O.check_local(g.n, g.layers(), bad, lmds=[lmd])[lmd][0] > 0
