@@ -64,7 +64,8 @@ typedef struct hg_graph hg_graph;
 /* Uploads the graph (layer i of `layers` is the reference's GetGraph(i),
  * i.e. layers already in -o order) and allocates the working memory for all
  * lmd = 1..L.  offset_bits is 32 or 64.  Limits: 1 <= L <= 64,
- * 1 <= n, n * L < 2^32, total directed entries < 2^40.  stream is a
+ * 1 <= n < 2^31 - 1, n * L < 2^32, total directed entries < 2^40
+ * (HG_ERR_RANGE otherwise).  stream is a
  * hipStream_t (NULL: default stream).  Replaces the MultilayerGraph
  * hand-over into FCSyncLeft::Execute (FCSyncLeft.cpp:202-235: degs init and
  * valid init happen here and in hg_decompose). */
@@ -137,17 +138,26 @@ int hg_cell_density(hg_graph* g, double* density, int32_t* num_select_layer, int
 int hg_densest_vertices(hg_graph* g, uint32_t* out, int64_t cap, int64_t* count);
 
 /* Device time of the phases of the last hg_decompose / hg_densest in ms,
- * measured with HIP events on the handle's stream. */
+ * measured with HIP events on the handle's stream.  scan_ms and cascade_ms
+ * are measured only with phase timing on (hg_set_phase_timing), else -1;
+ * the reference prints the analogous split per peel (FCSyncLeft.cpp:
+ * 240-245, U_overall). */
 typedef struct hg_timing {
     double decompose_ms;  /* whole hg_decompose, first to last kernel */
-    double init_ms;       /* level-0 scan (degree init) */
-    double scan_ms;       /* sum of level scans k >= 1 */
-    double cascade_ms;    /* sum of removal cascades */
+    double init_ms;       /* step 0: level-0 pass (degree init) and its removal cascade */
+    double scan_ms;       /* sum of the scans of steps 1, 2, ...; -1 unless phase timing is on */
+    double cascade_ms;    /* sum of the cascades of steps 1, 2, ...; -1 unless phase timing is on */
     double densest_ms;    /* edge histogram + cell scoring of hg_densest */
     int64_t levels;       /* level iterations run */
     int64_t launches;     /* kernel launches in hg_decompose */
 } hg_timing;
 int hg_last_timing(hg_graph* g, hg_timing* out);
+
+/* Phase timing for later hg_decompose calls (on != 0): an event is recorded
+ * before every scan and every cascade launch, so hg_timing.scan_ms and
+ * cascade_ms are filled in.  The events add a little device time to each
+ * step (a few thousand steps on a 100M-vertex graph); off by default. */
+int hg_set_phase_timing(hg_graph* g, int32_t on);
 
 /* Graph facts for reporting. */
 int hg_graph_info(hg_graph* g, int64_t* n, int32_t* L, int64_t* directed_entries, int64_t* device_bytes);

@@ -97,6 +97,10 @@ struct hg_graph {
     std::vector<double> h_pow;
     std::vector<int64_t> h_len;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    // phase timing (hg_set_phase_timing): events before each step's scan,
+    // between its scan and cascade, and after the last cascade
+    bool phase_timing = false;
+    std::vector<hipEvent_t> pev;
 };
 
 namespace {
@@ -112,6 +116,8 @@ void release(hg_graph* g) {
         if (p) hgc::dfree(p);
     if (g->h_ctl) (void)hipHostFree(g->h_ctl);
     for (auto& e : g->ev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto& e : g->pev)
         if (e) (void)hipEventDestroy(e);
     delete g;
 }
@@ -386,10 +392,26 @@ const char* err_name(int32_t e) {
     }
 }
 
+// Event `i` of the phase-timing pool, created on first use.
+int phase_event(hg_graph* g, size_t i) {
+    while (g->pev.size() <= i) {
+        hipEvent_t e = nullptr;
+        HIP_TRY(hipEventCreate(&e));
+        g->pev.push_back(e);
+    }
+    return HG_OK;
+}
+
 // Launches step s: the scan (level-0 init at s = 0), which also decides
 // the step, then the cascade.  Kernels of instances that are done return at
-// once.
-void launch_step(hg_graph* g, uint32_t s) {
+// once.  With phase timing, events 2s and 2s+1 go before the scan and
+// before the cascade.
+int launch_step(hg_graph* g, uint32_t s) {
+    if (g->phase_timing) {
+        int rc = phase_event(g, 2 * size_t(s) + 1);
+        if (rc) return rc;
+        HIP_TRY(hipEventRecord(g->pev[2 * size_t(s)], g->stream));
+    }
     const uint32_t L = g->L;
     if (s == 0) {
         // level-0 counters per thread: 8 when the graph has at most 8 layers
@@ -414,7 +436,9 @@ void launch_step(hg_graph* g, uint32_t s) {
     }
     else
         hipLaunchKernelGGL(k_scan, dim3(g->G), dim3(kBlock), 0, g->stream, g->d_insts, g->d_states, g->d_ctl, L, s);
+    if (g->phase_timing) HIP_TRY(hipEventRecord(g->pev[2 * size_t(s) + 1], g->stream));
     hipLaunchKernelGGL(k_cascade, dim3(g->G), dim3(kBlock), 0, g->stream, g->d_insts, g->d_states, g->d_ctl, L, s);
+    return HG_OK;
 }
 
 int decompose_impl(hg_graph* g) {
@@ -447,7 +471,8 @@ int decompose_impl(hg_graph* g) {
 
     HIP_TRY(hipEventRecord(g->ev[0], g->stream));
     uint32_t s = 0;
-    launch_step(g, s++);
+    int lrc = launch_step(g, s++);
+    if (lrc) return lrc;
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(g->ev[1], g->stream));
     // Steps run in batches; the host only reads the active count between
@@ -456,7 +481,10 @@ int decompose_impl(hg_graph* g) {
     uint32_t batch = 4;
     const uint64_t step_bound = 4ull * (uint64_t(g->kcap) + 8) + 2ull * g->n + 64;
     for (;;) {
-        for (uint32_t b = 0; b < batch; ++b) launch_step(g, s++);
+        for (uint32_t b = 0; b < batch; ++b) {
+            lrc = launch_step(g, s++);
+            if (lrc) return lrc;
+        }
         HIP_TRY(hipGetLastError());
         int rc = sync_ctl(g);
         if (rc) return rc;
@@ -464,6 +492,11 @@ int decompose_impl(hg_graph* g) {
         if (g->h_ctl->active == 0) break;
         if (s > step_bound) return fail(HG_ERR_DEVICE, "step loop did not terminate");
         batch = std::min<uint32_t>(batch * 2, uint32_t(batch_max));
+    }
+    if (g->phase_timing) {
+        lrc = phase_event(g, 2 * size_t(s));
+        if (lrc) return lrc;
+        HIP_TRY(hipEventRecord(g->pev[2 * size_t(s)], g->stream));
     }
     // Cell lengths and kmax from the histogram of core indices.
     HIP_TRY(hipMemsetAsync(g->d_ihist, 0, sizeof(uint64_t) * size_t(g->kcap) * L, g->stream));
@@ -483,6 +516,25 @@ int decompose_impl(hg_graph* g) {
     g->timing.init_ms = ms_init;
     g->timing.scan_ms = -1;
     g->timing.cascade_ms = -1;
+    if (g->phase_timing) {
+        // step 0 (level-0 init and its cascade) is init_ms, as without
+        // phase timing; scans and cascades of steps 1..s-1 are summed
+        double scan = 0, cas = 0;
+        for (uint32_t t = 0; t < s; ++t) {
+            float a = 0, b = 0;
+            HIP_TRY(hipEventElapsedTime(&a, g->pev[2 * size_t(t)], g->pev[2 * size_t(t) + 1]));
+            HIP_TRY(hipEventElapsedTime(&b, g->pev[2 * size_t(t) + 1], g->pev[2 * size_t(t) + 2]));
+            if (t == 0) {
+                ms_init = a + b;
+            } else {
+                scan += a;
+                cas += b;
+            }
+        }
+        g->timing.init_ms = ms_init;
+        g->timing.scan_ms = scan;
+        g->timing.cascade_ms = cas;
+    }
     g->timing.levels = int64_t(s);
     g->timing.launches = int64_t(s) * 2 + 3;
 
@@ -702,6 +754,12 @@ int hg_decompose(hg_graph* g) {
     } catch (const std::bad_alloc&) {
         return fail(HG_ERR_NOMEM, "host allocation failed in hg_decompose");
     }
+}
+
+int hg_set_phase_timing(hg_graph* g, int32_t on) {
+    if (!g) return fail(HG_ERR_ARG, "graph is NULL");
+    g->phase_timing = on != 0;
+    return HG_OK;
 }
 
 int hg_core_index(hg_graph* g, int32_t* core_index) {

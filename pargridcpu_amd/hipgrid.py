@@ -100,6 +100,7 @@ SIGNATURES = {
     "hg_cell_density": ([_P, C.POINTER(C.c_double), C.POINTER(C.c_int32), C.c_int32], C.c_int),
     "hg_densest_vertices": ([_P, C.POINTER(C.c_uint32), C.c_int64, C.POINTER(C.c_int64)], C.c_int),
     "hg_last_timing": ([_P, C.POINTER(Timing)], C.c_int),
+    "hg_set_phase_timing": ([_P, C.c_int32], C.c_int),
     "hg_graph_info": ([_P, C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_int64)],
                       C.c_int),
     "hg_graph_destroy": ([_P], C.c_int),
@@ -250,6 +251,11 @@ class Engine:
         t = Timing()
         _check(lib().hg_last_timing(self._h, C.byref(t)))
         return t
+
+    def set_phase_timing(self, on: bool = True) -> None:
+        """Events around every scan and cascade of later decompositions, so
+        timing().scan_ms / cascade_ms are measured (else -1)."""
+        _check(lib().hg_set_phase_timing(self._h, 1 if on else 0))
 
     def info(self) -> dict:
         n, L, D, b = C.c_int64(), C.c_int32(), C.c_int64(), C.c_int64()

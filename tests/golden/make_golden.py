@@ -25,7 +25,15 @@ and, per graph, <name>_wdsci.npz from CoreIndex::WdsCoreIndex
   ci_<beta>_printed  the printed answer: density (6 significant digits),
                      res_k, res_lmd
 
-Usage: python tests/golden/make_golden.py [--wdsci-only]
+and savecore.npz from the reference's own driver (oracle/_ref/firmcore_ref,
+main.cpp) run as `-m kpe -k K -lmd L`, which writes the cell's vertex
+labels through FCTree::saveCoreToLocal (Core/FCTree.cpp:138-153,
+main.cpp:213-223):
+  <case>_header   the file's first line
+  <case>_labels   uint64, the labels in the file's order
+  <case>_k, <case>_lmd, <case>_dataset
+
+Usage: python tests/golden/make_golden.py [--wdsci-only | --savecore-only]
 
 Run after __graft_entry__.build().
 """
@@ -48,6 +56,7 @@ sys.path.insert(0, ROOT)
 from pargridcpu_amd.graph import C1, MultilayerGraph  # noqa: E402
 
 HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+REF_DRIVER = os.path.join(ROOT, "oracle", "_ref", "firmcore_ref")
 BETAS = (1.0, 0.5, 2.0)
 
 # name -> generator spec (seeded; see include/hg_host.h)
@@ -152,13 +161,7 @@ LOADER_LAYERS = [
 
 def make_loader_fixture(workdir: str) -> dict:
     ds = os.path.join(workdir, "loader")
-    os.makedirs(ds, exist_ok=True)
-    with open(os.path.join(ds, "mlg.conf"), "w") as f:
-        for i in range(len(LOADER_LAYERS)):
-            f.write(f"l{i}.txt\n")
-    for i, text in enumerate(LOADER_LAYERS):
-        with open(os.path.join(ds, f"l{i}.txt"), "w") as f:
-            f.write(text)
+    write_loader_dataset(ds)
     pre = os.path.join(workdir, "loader_lpe")
     run_harness("lpe", ds, pre)
     labels = np.fromfile(pre + ".labels", dtype=np.uint64)
@@ -187,7 +190,66 @@ def make_wdsci_fixture(name: str, spec, workdir: str) -> dict:
     return out
 
 
+# (case, dataset, k, lmd) for savecore.npz; "loader" is the hand-written
+# edge-list case below, the others are generated graphs written in the
+# reference's text format (labels = generator ids, reference ids = first
+# appearance).
+SAVECORE_CASES = [("g2000_L5_k2_lmd2", "g2000_L5", 2, 2), ("g2000_L5_kmax_lmd1", "g2000_L5", 0, 1),
+                  ("c1_k3_lmd4", "c1", 3, 4), ("loader_k2_lmd1", "loader", 2, 1)]
+
+
+def write_loader_dataset(ds: str) -> None:
+    os.makedirs(ds, exist_ok=True)
+    with open(os.path.join(ds, "mlg.conf"), "w") as f:
+        for i in range(len(LOADER_LAYERS)):
+            f.write(f"l{i}.txt\n")
+    for i, text in enumerate(LOADER_LAYERS):
+        with open(os.path.join(ds, f"l{i}.txt"), "w") as f:
+            f.write(text)
+
+
+def make_savecore_fixture(workdir: str) -> dict:
+    """Runs the reference driver's kpe query per case (cwd run/, datasets in
+    ../dataset/, the file lands in ../output/) and keeps its output file."""
+    root = os.path.join(workdir, "savecore")
+    for sub in ("dataset", "run", "output"):
+        os.makedirs(os.path.join(root, sub), exist_ok=True)
+    out = {}
+    for case, dataset, k, lmd in SAVECORE_CASES:
+        ds = os.path.join(root, "dataset", dataset)
+        if not os.path.exists(ds):
+            if dataset == "loader":
+                write_loader_dataset(ds)
+            else:
+                MultilayerGraph.generate(GRAPHS[dataset]).write_text(ds)
+        if k == 0:  # the largest k of this lmd, from the golden index
+            k = int(np.load(os.path.join(HERE, dataset + ".npz"))["core_index"][lmd - 1].max())
+        r = subprocess.run([REF_DRIVER, "-d", dataset, "-m", "kpe", "-k", str(k), "-lmd", str(lmd)],
+                           cwd=os.path.join(root, "run"), capture_output=True, text=True, timeout=600)
+        if r.returncode != 0:
+            raise RuntimeError(f"reference driver failed on {case}: {r.stderr[-2000:]}")
+        path = os.path.join(root, "output", f"{dataset}_k_{k}_lmd_{lmd}_serial.txt")
+        with open(path) as f:
+            lines = f.read().splitlines()
+        out[case + "_dataset"] = np.array(dataset)
+        out[case + "_k"] = np.int64(k)
+        out[case + "_lmd"] = np.int64(lmd)
+        out[case + "_header"] = np.array(lines[0])
+        out[case + "_labels"] = np.array([int(x) for x in lines[1:]], dtype=np.uint64)
+    return out
+
+
 def main() -> int:
+    if "--savecore-only" in sys.argv:
+        if not os.path.exists(REF_DRIVER):
+            print("oracle/_ref/firmcore_ref is not built (make -C oracle ref)", file=sys.stderr)
+            return 2
+        with tempfile.TemporaryDirectory(prefix="hg_golden_") as work:
+            fx = make_savecore_fixture(work)
+        np.savez_compressed(os.path.join(HERE, "savecore.npz"), **fx)
+        print("wrote savecore: " + ", ".join(f"{c}: {fx[c + '_header']} ({fx[c + '_labels'].size} labels)"
+                                            for c, *_ in SAVECORE_CASES))
+        return 0
     if not os.path.exists(HARNESS):
         print("oracle/_ref/ref_harness is not built (make -C oracle ref)", file=sys.stderr)
         return 2
@@ -207,6 +269,8 @@ def main() -> int:
         fx = make_loader_fixture(work)
         np.savez_compressed(os.path.join(HERE, "loader.npz"), **fx)
         print(f"wrote loader: labels={fx['labels'].tolist()} index={fx['core_index'].tolist()}")
+        fx = make_savecore_fixture(work)
+        np.savez_compressed(os.path.join(HERE, "savecore.npz"), **fx)
     return 0
 
 

@@ -44,10 +44,13 @@ def test_create_argument_errors_without_device():
     assert lib.hg_graph_create(arr, 0, 10, 64, None, C.byref(out)) == -4
     assert lib.hg_graph_create(arr, 65, 10, 64, None, C.byref(out)) == -4
     assert lib.hg_graph_create(arr, 8, 1 << 29, 64, None, C.byref(out)) == -4  # n * L >= 2^32
+    assert lib.hg_graph_create(arr, 1, (1 << 31) - 1, 64, None, C.byref(out)) == -4  # n >= 2^31 - 1
+    assert b"2^31" in lib.hg_last_error()
     assert lib.hg_graph_create(arr, 1, 0, 64, None, C.byref(out)) == -1
     assert lib.hg_graph_create(arr, 1, 10, 16, None, C.byref(out)) == -1
     assert b"offset_bits" in lib.hg_last_error()
     assert lib.hg_decompose(None) == -1
+    assert lib.hg_set_phase_timing(None, 1) == -1
     assert lib.hg_graph_destroy(None) == -1
 
 

@@ -212,3 +212,47 @@ def test_concurrent_batches_from_threads():
     for t in ts:
         t.join()
     assert not bad, bad
+
+
+@pytest.mark.timeout(600)
+def test_full_batched_workload_against_oracle(capsys):
+    """The full batched workload: 256 graphs (n log-uniform in [1e3, 2e5], L
+    in {2..16}, average degree in [2, 20], gamma in [2.1, 3.0]; 744M directed
+    entries) in one launch with the default shape and once more through the
+    captured-graph replay; every graph's core index and cells equal the
+    oracle's CoreIndex restatement (graphs spread over host threads)."""
+    import concurrent.futures as cf
+    import time
+
+    from pargridcpu_amd.graph import batch_workload
+
+    def note(msg):
+        with capsys.disabled():
+            print(msg, flush=True)
+
+    t0 = time.perf_counter()
+    gs = [MultilayerGraph.generate(s) for s in batch_workload(256)]
+    note(f"case batch full workload: 256 graphs, D={sum(g.directed_entries() for g in gs)}, "
+         f"generated in {time.perf_counter() - t0:.0f} s")
+    with Batch(gs) as b:
+        b.decompose()
+        first = [b.core_index(i) for i in range(len(gs))]
+        cells = [b.cells(i) for i in range(len(gs))]
+        b.decompose()
+        for i in range(len(gs)):
+            assert np.array_equal(b.core_index(i), first[i]), f"replay differs, graph {i}"
+    note(f"  batch done at {time.perf_counter() - t0:.0f} s; oracle over 16 host threads")
+    order = sorted(range(len(gs)), key=lambda i: -gs[i].directed_entries() * gs[i].L)  # largest first
+
+    def one(i):
+        return i, O.coreindex(gs[i].n, gs[i].layers())
+
+    with cf.ThreadPoolExecutor(max_workers=16) as ex:
+        for i, ref in ex.map(one, order):
+            assert np.array_equal(first[i], ref), f"graph {i}"
+            cl, kmax = cells[i]
+            assert np.array_equal(kmax, ref.max(axis=1)), f"graph {i}"
+            for l in range(gs[i].L):
+                h = np.bincount(ref[l], minlength=int(kmax[l]) + 1)
+                assert np.array_equal(cl[l, :kmax[l] + 1], np.cumsum(h[::-1])[::-1]), f"graph {i} lmd {l + 1}"
+    note(f"  256 graphs exact at {time.perf_counter() - t0:.0f} s")

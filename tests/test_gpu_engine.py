@@ -249,3 +249,27 @@ def test_wds_coreindex_float_counter_saturation():
     assert [(w.res_k, w.res_lmd) for w in per] == [(n - 1, 1), (n - 1, 2)]
     assert (np.float32(ans.max_density), ans.res_k, ans.res_lmd) == (want, n - 1, 1)  # lmd = L takes no part
 
+
+
+def test_phase_timing():
+    """With phase timing on, the scan and cascade sums are measured and add
+    up (with level 0) to about the whole decomposition; off, they are -1;
+    results are the same either way."""
+    mg = MultilayerGraph.generate(small_spec(20000, 6, 60000, 21))
+    say("case phase timing n=20000 L=6")
+    ref = O.coreindex(mg.n, mg.layers())
+    with Engine.from_graph(mg) as eng:
+        eng.decompose()
+        t = eng.timing()
+        assert t.scan_ms == -1 and t.cascade_ms == -1
+        eng.set_phase_timing(True)
+        eng.decompose()
+        t = eng.timing()
+        assert np.array_equal(eng.core_index(), ref)
+        assert t.scan_ms > 0 and t.cascade_ms > 0 and t.init_ms > 0
+        parts = t.init_ms + t.scan_ms + t.cascade_ms
+        assert parts <= t.decompose_ms * 1.001 + 0.01
+        assert parts >= 0.5 * t.decompose_ms
+        eng.set_phase_timing(False)
+        eng.decompose()
+        assert eng.timing().scan_ms == -1

@@ -172,3 +172,65 @@ def test_full_size_c2_properties():
     for k in (0, 1, int(kmax[0])):
         s = vtx[off[i * stride + k]:off[i * stride + k + 1]]
         assert np.array_equal(s, np.nonzero(ref[i] == k)[0].astype(np.uint32))
+
+
+def _driver_root(tmp_path, dataset):
+    """Lays out <tmp>/dataset/<dataset>, <tmp>/run (cwd), <tmp>/output: the
+    reference's relative paths (main.cpp:98, Core/FCTree.cpp:141)."""
+    for sub in ("dataset", "run", "output"):
+        (tmp_path / sub).mkdir(exist_ok=True)
+    ds = tmp_path / "dataset" / dataset
+    if dataset == "loader":
+        ds.mkdir()
+        (ds / "mlg.conf").write_text("".join(f"l{i}.txt\n" for i in range(len(LOADER_LAYERS))))
+        for i, text in enumerate(LOADER_LAYERS):
+            (ds / f"l{i}.txt").write_text(text)
+    else:
+        MultilayerGraph.generate(GRAPHS[dataset]).write_text(str(ds))
+    return tmp_path
+
+
+@pytest.mark.parametrize("case", ["g2000_L5_k2_lmd2", "g2000_L5_kmax_lmd1", "c1_k3_lmd4", "loader_k2_lmd1"])
+def test_driver_query_file_matches_reference(tmp_path, case):
+    """`-m HipGrid -k K -lmd L` writes the cell's vertex labels; the same
+    header and the same set of labels as the reference driver's
+    `-m kpe -k K -lmd L` file (FCTree::saveCoreToLocal, Core/FCTree.cpp:
+    138-153, main.cpp:213-223), which lists them in its core[] order."""
+    fx = fixture("savecore")
+    dataset, k, lmd = str(fx[case + "_dataset"]), int(fx[case + "_k"]), int(fx[case + "_lmd"])
+    root = _driver_root(tmp_path, dataset)
+    exe = os.path.join(ROOT, "pargridcpu_amd", "bin", "hipgrid")
+    say(f"case driver query {case}")
+    r = subprocess.run([exe, "-d", dataset, "-m", "HipGrid", "-k", str(k), "-lmd", str(lmd)],
+                       cwd=str(root / "run"), capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    lines = (root / "output" / f"{dataset}_k_{k}_lmd_{lmd}_HipGrid.txt").read_text().splitlines()
+    assert lines[0] == str(fx[case + "_header"])
+    got = np.array([int(x) for x in lines[1:]], dtype=np.uint64)
+    want = fx[case + "_labels"]
+    assert got.size == want.size
+    assert np.array_equal(np.sort(got), np.sort(want))
+    # a cell above kmax: the reference's "No statisfy result" line
+    r = subprocess.run([exe, "-d", dataset, "-m", "HipGrid", "-k", "100000", "-lmd", str(lmd)],
+                       cwd=str(root / "run"), capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0 and "No statisfy result" in r.stdout
+
+
+@pytest.mark.parametrize("name", ["g2000_L5", "g4000_L4_loguni"])
+def test_driver_mem_shells_match_reference(tmp_path, name):
+    """`-m HipGridMem` prints every (k, lmd) shell's size; the reference's
+    CoreMem keeps the same shells as father->diff_res
+    (CoreParallel/FCSyncMem.cpp:42-51), whose vertices are exactly those of
+    golden index k (the fixture script checks CoreMem against lpe)."""
+    fx = fixture(name)
+    root = _driver_root(tmp_path, name)
+    exe = os.path.join(ROOT, "pargridcpu_amd", "bin", "hipgrid")
+    say(f"case driver HipGridMem {name}")
+    r = subprocess.run([exe, "-d", name, "-m", "HipGridMem"], cwd=str(root / "run"), capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stderr
+    got = {(int(k), int(l)): int(s) for k, l, s in re.findall(r"shell k = (\d+) lmd = (\d+) size = (\d+)", r.stdout)}
+    ci = fx["core_index"]
+    want = {(k, i + 1): int(np.count_nonzero(ci[i] == k)) for i in range(ci.shape[0])
+            for k in range(1, int(ci[i].max()) + 1)}
+    assert got == want

@@ -1,6 +1,8 @@
 """GPU tests of the request loop (hg_server_*): every answer equals the
-single-graph engine's on that graph alone, whatever the mix, batching,
-lane count or memory budget; admission respects the budget."""
+oracle's (CoreIndex restatement for the core index, WdsList restatement for
+densest requests) and the single-graph engine's on that graph alone,
+whatever the mix, batching, lane count or memory budget; admission respects
+the budget."""
 import dataclasses
 import sys
 import threading
@@ -9,7 +11,8 @@ import time
 import numpy as np
 import pytest
 
-from pargridcpu_amd.graph import MultilayerGraph, batch_spec
+from oracle import oracle as O
+from pargridcpu_amd.graph import MultilayerGraph, batch_spec, large_request_spec
 from pargridcpu_amd.hipgrid import REQ_DECOMPOSE, REQ_DENSEST, Engine, HgError, Server
 from tests import graphs as G
 
@@ -36,17 +39,40 @@ def alone(mg, kind, beta):
     return ci, kmax, w
 
 
-def check(srv, rid, mg, kind, beta):
+_ORACLE = {}
+
+
+def oracle_answer(mg, kind, beta):
+    """Core index from the oracle's CoreIndex restatement (one lmd per host
+    thread) and, for densest requests, the oracle's WdsList result; cached
+    per graph."""
+    key = (id(mg), kind, beta)
+    if key not in _ORACLE:
+        ci = O.coreindex_parallel(mg.n, mg.layers())
+        w = O.wds(mg.n, mg.layers(), beta) if kind == REQ_DENSEST else None
+        _ORACLE[key] = (mg, ci, w)  # mg kept so id() stays unique
+    return _ORACLE[key][1:]
+
+
+def check(srv, rid, mg, kind, beta, with_alone=True):
     info = srv.wait(rid)
     assert info.status == 0
     ci, kmax = srv.result(rid)
-    eci, ekmax, ew = alone(mg, kind, beta)
-    assert np.array_equal(ci, eci), f"request {rid}"
-    assert np.array_equal(kmax, ekmax)
+    oci, ow = oracle_answer(mg, kind, beta)
+    assert np.array_equal(ci, oci), f"request {rid}: core index differs from the oracle"
+    assert np.array_equal(kmax, oci.max(axis=1))
     if kind == REQ_DENSEST:
         w = info.wds
-        assert np.float32(w.max_density) == np.float32(ew.max_density)
-        assert (w.res_k, w.res_lmd, w.res_len) == (ew.res_k, ew.res_lmd, ew.res_len)
+        assert np.float32(w.max_density) == ow["max_density"], f"request {rid}"
+        assert (w.res_k, w.res_lmd, w.res_len) == (ow["res_k"], ow["res_lmd"], ow["res_len"])
+    if with_alone:
+        eci, ekmax, ew = alone(mg, kind, beta)
+        assert np.array_equal(ci, eci), f"request {rid}"
+        assert np.array_equal(kmax, ekmax)
+        if kind == REQ_DENSEST:
+            w = info.wds
+            assert np.float32(w.max_density) == np.float32(ew.max_density)
+            assert (w.res_k, w.res_lmd, w.res_len) == (ew.res_k, ew.res_lmd, ew.res_len)
     return info
 
 
@@ -129,3 +155,28 @@ def test_concurrent_densest_batches():
         ids = [srv.submit(g, REQ_DENSEST, 1.0) for g in gs]
         for rid, g in zip(ids, gs):
             check(srv, rid, g, REQ_DENSEST, 1.0)
+
+
+def test_large_requests_in_the_mix():
+    """The serving workload's large request (n = 5M, L = 12, 60M sampled
+    edges) twice, submitted among small decomposition and densest requests
+    with large_entries below its size: the large ones run alone on the
+    single-graph engine while the small ones are batched around them; every
+    answer equals the oracle's."""
+    large = MultilayerGraph.generate(large_request_spec(0))
+    small = request_graphs(16, 0xD6)
+    say(f"case server large requests n={large.n} L={large.L} D={large.directed_entries()} among 17 small")
+    say("  oracle for the large graph (one lmd per host thread)")
+    oracle_answer(large, REQ_DECOMPOSE, 1.0)
+    with Server(lanes=3, max_batch=8, min_batch=2, large_entries=2_000_000) as srv:
+        plan = [(g, REQ_DENSEST if i % 4 == 1 else REQ_DECOMPOSE, 1.0) for i, g in enumerate(small)]
+        plan.insert(3, (large, REQ_DECOMPOSE, 1.0))
+        plan.insert(11, (large, REQ_DECOMPOSE, 1.0))
+        ids = [srv.submit(g, k, b) for g, k, b in plan]
+        sizes = {}
+        for rid, (g, k, b) in zip(ids, plan):
+            info = check(srv, rid, g, k, b, with_alone=g is not large)
+            sizes.setdefault(g is large, []).append(info.batch_size)
+        assert sizes[True] == [1, 1]
+        st = srv.stats()
+        assert st["completed"] == len(plan) and st["failed"] == 0

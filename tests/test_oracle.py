@@ -279,3 +279,23 @@ def test_local_check_accepts_reference_and_rejects_off_by_one(name):
             bad = ref.copy()
             bad[lmd - 1, zeros[0]] = 1
             assert O.check_local(g.n, g.layers(), bad, lmds=[lmd])[lmd][0] > 0
+
+
+def test_savecore_fixture_is_the_golden_cell():
+    """The reference driver's saveCoreToLocal files (savecore.npz) hold
+    exactly S(k, lmd) of the golden index, by label, with the length in the
+    header."""
+    fx = load_fixture("savecore")
+    cases = sorted({k[:-len("_header")] for k in fx if k.endswith("_header")})
+    assert len(cases) == 4
+    for case in cases:
+        dataset, k, lmd = str(fx[case + "_dataset"]), int(fx[case + "_k"]), int(fx[case + "_lmd"])
+        labels = fx[case + "_labels"]
+        assert str(fx[case + "_header"]) == f"########## k_{k}_lmd_{lmd}_length_{labels.size}"
+        if dataset == "loader":
+            lf = load_fixture("loader")
+            ids = np.flatnonzero(lf["core_index"][lmd - 1] >= k)
+            want = lf["labels"][ids]
+        else:
+            want = np.flatnonzero(load_fixture(dataset)["core_index"][lmd - 1] >= k).astype(np.uint64)
+        assert np.array_equal(np.sort(labels), np.sort(want)), case
