@@ -129,6 +129,23 @@ int hg_densest(hg_graph* g, double beta, hg_wds* out);
  * Requires hg_decompose first. */
 int hg_densest_coreindex(hg_graph* g, double beta, hg_wds* answer, hg_wds* per_lmd);
 
+/* hg_densest without any device or pinned-memory allocation.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * during the call (code around HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same results as hg_densest, and
+ * hg_cell_density / hg_densest_vertices report on it the same way.  The
+ * pass's edge histograms, cell lengths, scores and pow table live in one
+ * fixed device array of HG_NOALLOC_SCRATCH_BYTES per GPU, held until the
+ * process ends; a graph needs 8 * L * (kmax + 1) + 28 * (kmax + 1) + 8 * L
+ * bytes of it (plus alignment), kmax the largest core index of any lmd, and
+ * a larger need fails with HG_ERR_RANGE.  Host copies are plain copies of
+ * the caller's memory (no staging buffers).  Every call on one GPU shares
+ * that GPU's array: calls that overlap on one GPU (from two threads, on any
+ * handles) race and must be serialised by the caller.  Requires
+ * hg_decompose first. */
+#define HG_NOALLOC_SCRATCH_BYTES (256 * 1024)
+int hg_densest_noalloc(hg_graph* g, double beta, hg_wds* out);
+
 /* Per-cell densities of the last hg_densest: density[(lmd-1)*cap + k] for
  * k < cap (0 where the cell is empty or k = 0).  Either output may be NULL. */
 int hg_cell_density(hg_graph* g, double* density, int32_t* num_select_layer, int32_t cap);

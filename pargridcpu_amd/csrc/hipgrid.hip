@@ -26,6 +26,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -568,21 +569,48 @@ std::vector<int64_t> cell_lengths(const hg_graph* g, uint32_t i) {
     return len;
 }
 
-int densest_impl(hg_graph* g, double beta, hg_wds* out) {
-    HIP_TRY(hipSetDevice(g->device));
+// Device buffers of one densest pass: per-layer edge histograms [L][kc]
+// (partial results), and per cell the lengths, scores and chosen layer
+// counts, plus the pow(l+1, beta) table.
+struct DenseBufs {
+    uint64_t* hist;
+    double* density;
+    int32_t* sel;
+    int64_t* cell_len;
+    double* pow;
+};
+
+// Host <-> device copies of the densest pass: through the pinned staging
+// pool, or (no_alloc) as plain copies of the caller's pageable memory, which
+// allocate nothing.
+int dense_h2d(hgc::Staging& io, bool no_alloc, void* dst, const void* src, size_t bytes, hipStream_t st) {
+    if (no_alloc) {
+        HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipStreamSynchronize(st));  // src may be reused on return
+    } else {
+        HIP_TRY(io.h2d(dst, src, bytes));
+    }
+    return HG_OK;
+}
+
+int dense_d2h(hgc::Staging& io, bool no_alloc, void* dst, const void* src, size_t bytes, hipStream_t st) {
+    if (no_alloc) {
+        HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+    } else {
+        HIP_TRY(io.d2h(dst, src, bytes));
+    }
+    return HG_OK;
+}
+
+int densest_run(hg_graph* g, double beta, hg_wds* out, const DenseBufs& b, bool no_alloc) {
     const uint32_t L = g->L;
     std::vector<double>& pw = g->h_pow;
     pw.assign(L, 0.0);
     for (uint32_t l = 0; l < L; ++l) pw[l] = std::pow(double(l + 1), beta);
     hgc::Staging io(g->stream);
-    HIP_TRY(io.h2d(g->d_pow, pw.data(), sizeof(double) * L));
-    int32_t kmax_all = 0;
-    for (uint32_t i = 0; i < L; ++i) kmax_all = std::max(kmax_all, g->h_kmax[i]);
-    const size_t kh = size_t(kmax_all) + 1;
-    if (!g->hist) {
-        hipError_t e = dalloc(&g->hist, size_t(L) * g->kcap);
-        if (e != hipSuccess) return fail(HG_ERR_NOMEM, "densest histogram allocation failed");
-    }
+    int rc = dense_h2d(io, no_alloc, b.pow, pw.data(), sizeof(double) * L, g->stream);
+    if (rc) return rc;
     g->h_density.assign(size_t(g->kcap) * L, 0.0);
     g->h_sel.assign(size_t(g->kcap) * L, 0);
     HIP_TRY(hipEventRecord(g->ev[0], g->stream));
@@ -592,18 +620,22 @@ int densest_impl(hg_graph* g, double beta, hg_wds* out) {
         const int32_t km = g->h_kmax[i];
         if (km < 1) continue;
         const uint32_t kc = uint32_t(km) + 1;
-        HIP_TRY(hipMemsetAsync(g->hist, 0, sizeof(uint64_t) * size_t(L) * kc, g->stream));
+        HIP_TRY(hipMemsetAsync(b.hist, 0, sizeof(uint64_t) * size_t(L) * kc, g->stream));
         hipLaunchKernelGGL(k_edge_hist, dim3(std::max(grid, 1u)), dim3(kHistBlock), 0, g->stream, g->insts[i], g->chunks,
-                           g->nchunks, g->hist, kc);
+                           g->nchunks, b.hist, kc);
         std::vector<int64_t>& len = g->h_len;
         len = cell_lengths(g, i);
-        HIP_TRY(io.h2d(g->d_cell_len, len.data(), sizeof(int64_t) * len.size()));
+        if ((rc = dense_h2d(io, no_alloc, b.cell_len, len.data(), sizeof(int64_t) * len.size(), g->stream))) return rc;
         const uint32_t cells = uint32_t(km);
-        hipLaunchKernelGGL(k_cell_score, dim3((cells + 63) / 64), dim3(64), 0, g->stream, g->hist, kc, L, uint32_t(km),
-                           g->d_cell_len, g->d_pow, g->d_density, g->d_sel);
+        hipLaunchKernelGGL(k_cell_score, dim3((cells + 63) / 64), dim3(64), 0, g->stream, b.hist, kc, L, uint32_t(km),
+                           b.cell_len, b.pow, b.density, b.sel);
         HIP_TRY(hipGetLastError());
-        HIP_TRY(io.d2h(g->h_density.data() + size_t(i) * g->kcap, g->d_density, sizeof(double) * (size_t(km) + 1)));
-        HIP_TRY(io.d2h(g->h_sel.data() + size_t(i) * g->kcap, g->d_sel, sizeof(int32_t) * (size_t(km) + 1)));
+        if ((rc = dense_d2h(io, no_alloc, g->h_density.data() + size_t(i) * g->kcap, b.density,
+                            sizeof(double) * (size_t(km) + 1), g->stream)))
+            return rc;
+        if ((rc = dense_d2h(io, no_alloc, g->h_sel.data() + size_t(i) * g->kcap, b.sel,
+                            sizeof(int32_t) * (size_t(km) + 1), g->stream)))
+            return rc;
         g->h_density[size_t(i) * g->kcap] = 0.0;
         g->h_sel[size_t(i) * g->kcap] = 0;
     }
@@ -612,7 +644,6 @@ int densest_impl(hg_graph* g, double beta, hg_wds* out) {
     float ms = 0;
     HIP_TRY(hipEventElapsedTime(&ms, g->ev[0], g->ev[1]));
     g->timing.densest_ms = ms;
-    (void)kh;
     // Argmax exactly as FCTree::WeightDenestSubgraph with main.cpp's float
     // running max: pre-order = lambda-major, k ascending; update on >=.
     hg_wds r{};
@@ -635,6 +666,68 @@ int densest_impl(hg_graph* g, double beta, hg_wds* out) {
     g->densest_done = true;
     if (out) *out = r;
     return HG_OK;
+}
+
+int densest_impl(hg_graph* g, double beta, hg_wds* out) {
+    HIP_TRY(hipSetDevice(g->device));
+    if (!g->hist) {
+        hipError_t e = dalloc(&g->hist, size_t(g->L) * g->kcap);
+        if (e != hipSuccess) return fail(HG_ERR_NOMEM, "densest histogram allocation failed");
+    }
+    return densest_run(g, beta, out, DenseBufs{g->hist, g->d_density, g->d_sel, g->d_cell_len, g->d_pow}, false);
+}
+
+// ------------------------------------------------------- no-allocation densest
+// hg_densest_noalloc keeps the pass's buffers in one file-scope device array
+// (each GPU holds its own copy until the process ends).  In host code the
+// name `scratch` is only a host-side stand-in: kernels, memsets and copies
+// take the per-device address from hipGetSymbolAddress, looked up once per
+// device with that device current.
+static __device__ __align__(256) uint8_t scratch[HG_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_address(int device, uint8_t** out) {
+    if (device < 0 || device >= kMaxDevices) return fail(HG_ERR_RANGE, "device ordinal above 63");
+    std::lock_guard<std::mutex> lk(g_scratch_mutex);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(HG_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[device] = p;
+    }
+    *out = static_cast<uint8_t*>(g_scratch_ptr[device]);
+    return HG_OK;
+}
+
+size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
+
+int densest_noalloc_impl(hg_graph* g, double beta, hg_wds* out) {
+    HIP_TRY(hipSetDevice(g->device));
+    int32_t km = 0;
+    for (uint32_t i = 0; i < g->L; ++i) km = std::max(km, g->h_kmax[i]);
+    const size_t kc = size_t(km) + 1;
+    const size_t o_hist = 0;
+    const size_t o_dens = align_up(o_hist + sizeof(uint64_t) * size_t(g->L) * kc);
+    const size_t o_sel = align_up(o_dens + sizeof(double) * kc);
+    const size_t o_len = align_up(o_sel + sizeof(int32_t) * kc);
+    const size_t o_pow = align_up(o_len + sizeof(int64_t) * kc);
+    const size_t need = o_pow + sizeof(double) * g->L;
+    if (need > size_t(HG_NOALLOC_SCRATCH_BYTES))
+        return fail(HG_ERR_RANGE, "hg_densest_noalloc: the pass needs " + std::to_string(need) +
+                                      " bytes of scratch (8 * L * (kmax + 1) for the edge histograms plus 28 * "
+                                      "(kmax + 1) + 8 * L), above HG_NOALLOC_SCRATCH_BYTES = " +
+                                      std::to_string(HG_NOALLOC_SCRATCH_BYTES));
+    uint8_t* base = nullptr;
+    int rc = scratch_address(g->device, &base);
+    if (rc) return rc;
+    const DenseBufs b{reinterpret_cast<uint64_t*>(base + o_hist), reinterpret_cast<double*>(base + o_dens),
+                      reinterpret_cast<int32_t*>(base + o_sel), reinterpret_cast<int64_t*>(base + o_len),
+                      reinterpret_cast<double*>(base + o_pow)};
+    return densest_run(g, beta, out, b, true);
 }
 
 
@@ -753,6 +846,17 @@ int hg_decompose(hg_graph* g) {
         return decompose_impl(g);
     } catch (const std::bad_alloc&) {
         return fail(HG_ERR_NOMEM, "host allocation failed in hg_decompose");
+    }
+}
+
+int hg_densest_noalloc(hg_graph* g, double beta, hg_wds* out) {
+    if (!g) return fail(HG_ERR_ARG, "graph is NULL");
+    if (!g->decomposed) return fail(HG_ERR_STATE, "hg_decompose has not completed");
+    if (!std::isfinite(beta)) return fail(HG_ERR_ARG, "beta must be finite");
+    try {
+        return densest_noalloc_impl(g, beta, out);
+    } catch (const std::bad_alloc&) {
+        return fail(HG_ERR_NOMEM, "host allocation failed in hg_densest_noalloc");
     }
 }
 

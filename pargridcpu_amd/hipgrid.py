@@ -97,6 +97,7 @@ SIGNATURES = {
     "hg_shells": ([_P, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_uint32)], C.c_int),
     "hg_densest": ([_P, C.c_double, C.POINTER(Wds)], C.c_int),
     "hg_densest_coreindex": ([_P, C.c_double, C.POINTER(Wds), C.POINTER(Wds)], C.c_int),
+    "hg_densest_noalloc": ([_P, C.c_double, C.POINTER(Wds)], C.c_int),
     "hg_cell_density": ([_P, C.POINTER(C.c_double), C.POINTER(C.c_int32), C.c_int32], C.c_int),
     "hg_densest_vertices": ([_P, C.POINTER(C.c_uint32), C.c_int64, C.POINTER(C.c_int64)], C.c_int),
     "hg_last_timing": ([_P, C.POINTER(Timing)], C.c_int),
