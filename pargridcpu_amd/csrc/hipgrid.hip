@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -102,6 +103,27 @@ struct hg_graph {
     // between its scan and cascade, and after the last cascade
     bool phase_timing = false;
     std::vector<hipEvent_t> pev;
+    // row compaction (k_cmp_*): per instance two row buffers used in turn,
+    // plus shared scratch for the counts, prefix sums and chunk list
+    struct CmpBuf {
+        int64_t* off[2] = {nullptr, nullptr};
+        uint32_t* adj[2] = {nullptr, nullptr};
+        uint64_t adj_cap[2] = {0, 0};
+    };
+    std::vector<CmpBuf> cmp;
+    uint64_t* c_len = nullptr;       // [n+1] new row lengths, then cursors
+    uint64_t* c_nch = nullptr;       // [n+1] chunks per current row
+    uint64_t* c_item_off = nullptr;  // [n+1]
+    uint64_t* c_items = nullptr;
+    uint64_t c_items_cap = 0;
+    uint8_t* c_temp = nullptr;
+    size_t c_temp_bytes = 0;
+    unsigned long long* c_words = nullptr;
+    uint64_t compact_min = 0;  // compaction for graphs of at least this many entries (0: never)
+    uint64_t compact_ratio = 2;  // compact when alive entries <= rows' entries / this
+    int64_t cmp_bytes = 0;
+    std::vector<Inst> cur;     // the instances' current rows during hg_decompose
+    int64_t compactions = 0;   // in the last hg_decompose
 };
 
 namespace {
@@ -120,6 +142,14 @@ void release(hg_graph* g) {
         if (e) (void)hipEventDestroy(e);
     for (auto& e : g->pev)
         if (e) (void)hipEventDestroy(e);
+    for (auto& c : g->cmp)
+        for (int b = 0; b < 2; ++b) {
+            if (c.off[b]) hgc::dfree(c.off[b]);
+            if (c.adj[b]) hgc::dfree(c.adj[b]);
+        }
+    void* cptrs[] = {g->c_len, g->c_nch, g->c_item_off, g->c_items, g->c_temp, g->c_words};
+    for (void* p : cptrs)
+        if (p) hgc::dfree(p);
     delete g;
 }
 
@@ -281,6 +311,17 @@ int create_impl(const hg_layer_csr* layers, int32_t L, int64_t n, int32_t offset
     g->H = uint32_t(h_prep[kPrepHeavy]);
     g->pull = uint32_t(L) <= kPullMaxL && env_int("HG_PULL", 1) == 1 ? 1u : 0u;
     g->kcap = uint32_t(maxdeg + 2);
+    // Row compaction between levels (k_cmp_*), off by default: on the
+    // 100M-vertex workload it cuts the steps from 3229 to 2301 and their time
+    // by ~40 ms, but its passes over all n vertices cost more (598-636 vs
+    // 545 ms per decomposition).  HG_COMPACT=1 turns it on for graphs of 2^28
+    // entries or more, 2 for every graph (tests).
+    {
+        const char* cv = std::getenv("HG_COMPACT");
+        const int c = cv && *cv ? std::atoi(cv) : 0;
+        g->compact_min = c == 1 ? (1ull << 28) : c == 2 ? 1 : 0;
+        g->compact_ratio = uint64_t(std::max(2, env_int("HG_CMP_RATIO", 2)));
+    }
     // Level-kernel grid: 3 blocks per CU (the cascade's LDS allows 3; a sweep
     // on the 1M- and 100M-vertex workloads picked it over 2 and 4); HG_BLOCKS
     // overrides.
@@ -373,6 +414,8 @@ int create_impl(const hg_layer_csr* layers, int32_t L, int64_t n, int32_t offset
         in.lmagic = g->L == 1 ? 0xFFFFFFFFu : uint32_t((1ull << 32) / g->L);  // div_L needs magic < 2^32
         in.flags = env_int("HG_AGG", 1) == 2 ? 0u : kFlagAggregate;  // HG_AGG=2 disables combining
         in.heavy = uint32_t(env_int("HG_HEAVY", int(kHeavy)));
+        in.tail_heavy = uint32_t(env_int("HG_TAIL_HEAVY", int(in.heavy)));
+        in.tail_chunks = uint32_t(env_int("HG_TAIL_CHUNKS", int(g->G)));
         // deferred chunk size; the lists are sized for chunks of kMinDefer
         in.defer = std::min<uint32_t>(std::max<uint32_t>(uint32_t(env_int("HG_DEFER", int(kDefer))), kMinDefer),
                                       1u << 23);
@@ -389,6 +432,7 @@ int sync_ctl(hg_graph* g) {
 const char* err_name(int32_t e) {
     switch (e) {
         case kErrDefer: return "deferred chunk list overflow";
+        case kErrCompact: return "row compaction: copied rows do not match the degrees";
         default: return "unknown device error";
     }
 }
@@ -442,6 +486,142 @@ int launch_step(hg_graph* g, uint32_t s) {
     return HG_OK;
 }
 
+// Grows *p to hold at least `need` items (contents are not kept).
+template <typename T>
+int ensure_cap(hg_graph* g, T** p, uint64_t* cap, uint64_t need) {
+    if (*p && *cap >= need) return HG_OK;
+    if (*p) {
+        hgc::dfree(*p);
+        g->cmp_bytes -= int64_t(*cap * sizeof(T));
+        *p = nullptr;
+        *cap = 0;
+    }
+    const uint64_t want = need + need / 8 + 1024;
+    if (dalloc(p, size_t(want)) != hipSuccess) return fail(HG_ERR_NOMEM, "row compaction: device allocation failed");
+    *cap = want;
+    g->cmp_bytes += int64_t(want * sizeof(T));
+    return HG_OK;
+}
+
+uint32_t cmp_grid(const hg_graph* g, uint64_t items_per_thread_space) {
+    return uint32_t(std::max<uint64_t>(1, std::min<uint64_t>((items_per_thread_space + kBlock - 1) / kBlock,
+                                                             uint64_t(g->cus) * 16)));
+}
+
+// Sum over alive vertices of instance i of sum_l deg[v][l] (= its alive
+// entries at a level boundary); the arrays are written too when `write`.
+int cmp_count(hg_graph* g, const Inst& in, bool write, uint64_t* sum) {
+    HIP_TRY(hipMemsetAsync(g->c_words, 0, sizeof(unsigned long long) * 4, g->stream));
+    const uint32_t grid = cmp_grid(g, uint64_t(g->n) + 1);
+    if (write)
+        hipLaunchKernelGGL(k_cmp_count<true>, dim3(grid), dim3(kBlock), 0, g->stream, in, g->c_len, g->c_nch,
+                           g->c_words);
+    else
+        hipLaunchKernelGGL(k_cmp_count<false>, dim3(grid), dim3(kBlock), 0, g->stream, in, g->c_len, g->c_nch,
+                           g->c_words);
+    HIP_TRY(hipGetLastError());
+    unsigned long long w = 0;
+    HIP_TRY(hipMemcpyAsync(&w, g->c_words, sizeof w, hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    *sum = w;
+    return HG_OK;
+}
+
+// Rebuilds instance i's rows with only alive targets into its other buffer
+// (see k_cmp_*); `alive_entries` is the new total.
+int compact_rows(hg_graph* g, uint32_t i, uint64_t alive_entries) {
+    const size_t n1 = size_t(g->n) + 1;
+    hg_graph::CmpBuf& cb = g->cmp[i];
+    Inst& in = g->cur[i];
+    uint64_t sum = 0;
+    int rc = cmp_count(g, in, true, &sum);
+    if (rc) return rc;
+    if (sum != alive_entries) return fail(HG_ERR_DEVICE, "row compaction: alive entry count changed");
+    const int dst = in.off == cb.off[0] ? 1 : 0;
+    if (!cb.off[dst]) {
+        if (dalloc(&cb.off[dst], n1) != hipSuccess) return fail(HG_ERR_NOMEM, "row compaction: device allocation failed");
+        g->cmp_bytes += int64_t(8 * n1);
+    }
+    if ((rc = ensure_cap(g, &cb.adj[dst], &cb.adj_cap[dst], std::max<uint64_t>(sum, 1)))) return rc;
+    {
+        size_t tb = 0;
+        uint64_t* off_u = reinterpret_cast<uint64_t*>(cb.off[dst]);
+        HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, g->c_len, off_u, int(n1), g->stream));
+        if (tb > g->c_temp_bytes) {
+            if (g->c_temp) hgc::dfree(g->c_temp);
+            g->c_temp = nullptr;
+            if (dalloc(&g->c_temp, tb) != hipSuccess) return fail(HG_ERR_NOMEM, "row compaction: device allocation failed");
+            g->c_temp_bytes = tb;
+        }
+        size_t t1 = g->c_temp_bytes;
+        HIP_TRY(hipcub::DeviceScan::ExclusiveSum(g->c_temp, t1, g->c_len, off_u, int(n1), g->stream));
+        size_t t2 = g->c_temp_bytes;
+        HIP_TRY(hipcub::DeviceScan::ExclusiveSum(g->c_temp, t2, g->c_nch, g->c_item_off, int(n1), g->stream));
+    }
+    uint64_t nitems = 0;
+    HIP_TRY(hipMemcpyAsync(&nitems, g->c_item_off + g->n, sizeof nitems, hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    if ((rc = ensure_cap(g, &g->c_items, &g->c_items_cap, std::max<uint64_t>(nitems, 1)))) return rc;
+    hipLaunchKernelGGL(k_cmp_items, dim3(cmp_grid(g, g->n)), dim3(kBlock), 0, g->stream, in, cb.off[dst],
+                       g->c_item_off, g->c_items, g->c_len);
+    const uint32_t fill_grid = uint32_t(std::max<uint64_t>(
+        1, std::min<uint64_t>((nitems * kWave + kBlock - 1) / kBlock, uint64_t(g->cus) * 16)));
+    hipLaunchKernelGGL(k_cmp_fill, dim3(fill_grid), dim3(kBlock), 0, g->stream, in, g->c_items, nitems, cb.off[dst],
+                       g->c_len, cb.adj[dst], g->d_ctl);
+    hipLaunchKernelGGL(k_cmp_check, dim3(cmp_grid(g, g->n)), dim3(kBlock), 0, g->stream, in, cb.off[dst], g->c_len,
+                       g->d_ctl);
+    HIP_TRY(hipGetLastError());
+    in.off = cb.off[dst];
+    in.adj = cb.adj[dst];
+    in.flags |= kFlagCompact;
+    HIP_TRY(hipMemcpyAsync(g->d_insts + i, &in, sizeof(Inst), hipMemcpyHostToDevice, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));  // `in` may change before the next compaction
+    ++g->compactions;
+    return HG_OK;
+}
+
+// At a host sync before step s: compacts the rows of every instance at a
+// level boundary (nothing deferred to step s) whose alive entries have
+// fallen to half of its rows' entries at the last compaction (the graph's
+// entries at first).  The alive entries are counted only after the alive
+// vertices have fallen by a quarter since the last count.
+struct CmpTrack {
+    uint64_t next_check;  // count again once the alive vertices are at most this
+    uint64_t rows;        // entries of the current rows
+};
+
+int maybe_compact(hg_graph* g, uint32_t s, std::vector<CmpTrack>& tr) {
+    const uint32_t L = g->L;
+    std::vector<InstState>& st = g->h_states;
+    HIP_TRY(hipMemcpyAsync(st.data(), g->d_states, sizeof(InstState) * L, hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    for (uint32_t i = 0; i < L; ++i) {
+        const InstState& x = st[i];
+        if (x.dphase[(s + 1u) & 1u] == kPhaseDone) continue;  // the last step's decision
+        if (x.pdefer[s & 1u][0] != 0) continue;               // mid-level
+        const uint64_t alive = x.alive[0];
+        if (alive == 0 || alive > tr[i].next_check) continue;
+        tr[i].next_check = alive - alive / 4;
+        uint64_t a = 0;
+        const bool log = env_int("HG_CMP_LOG", 0) == 1;  // diagnostic
+        const auto t0 = std::chrono::steady_clock::now();
+        int rc = cmp_count(g, g->cur[i], false, &a);
+        if (rc) return rc;
+        if (log)
+            std::fprintf(stderr, "[hg] step %u lmd %u alive %llu entries %llu rows %llu count %.3f ms\n", s, i + 1,
+                         (unsigned long long)alive, (unsigned long long)a, (unsigned long long)tr[i].rows,
+                         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+        if (g->compact_ratio * a > tr[i].rows) continue;
+        const auto t1 = std::chrono::steady_clock::now();
+        if ((rc = compact_rows(g, i, a))) return rc;
+        if (log)
+            std::fprintf(stderr, "[hg]   compacted lmd %u in %.3f ms\n", i + 1,
+                         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t1).count());
+        tr[i].rows = a;
+    }
+    return HG_OK;
+}
+
 int decompose_impl(hg_graph* g) {
     HIP_TRY(hipSetDevice(g->device));
     const uint32_t L = g->L;
@@ -467,6 +647,18 @@ int decompose_impl(hg_graph* g) {
     ctl.active = int32_t(L);
     hgc::Staging io(g->stream);
     HIP_TRY(io.h2d(g->d_insts, g->insts.data(), sizeof(Inst) * L));
+    g->cur = g->insts;  // the rows every instance starts from
+    g->compactions = 0;
+    const bool compact = g->compact_min != 0 && g->D >= g->compact_min;
+    std::vector<CmpTrack> track(L, CmpTrack{uint64_t(g->n), g->D});
+    if (compact && !g->c_len) {
+        const size_t n1 = size_t(g->n) + 1;
+        if (g->cmp.empty()) g->cmp.resize(L);
+        if (dalloc(&g->c_len, n1) != hipSuccess || dalloc(&g->c_nch, n1) != hipSuccess ||
+            dalloc(&g->c_item_off, n1) != hipSuccess || dalloc(&g->c_words, 4) != hipSuccess)
+            return fail(HG_ERR_NOMEM, "row compaction: device allocation failed");
+        g->cmp_bytes += int64_t(24 * n1 + 32);
+    }
     HIP_TRY(io.h2d(g->d_states, st.data(), sizeof(InstState) * L));
     HIP_TRY(io.h2d(g->d_ctl, &ctl, sizeof(Control)));
 
@@ -491,6 +683,10 @@ int decompose_impl(hg_graph* g) {
         if (rc) return rc;
         if (g->h_ctl->error) return fail(HG_ERR_DEVICE, err_name(g->h_ctl->error));
         if (g->h_ctl->active == 0) break;
+        if (compact) {
+            rc = maybe_compact(g, s, track);
+            if (rc) return rc;
+        }
         if (s > step_bound) return fail(HG_ERR_DEVICE, "step loop did not terminate");
         batch = std::min<uint32_t>(batch * 2, uint32_t(batch_max));
     }
@@ -997,7 +1193,7 @@ int hg_graph_info(hg_graph* g, int64_t* n, int32_t* L, int64_t* directed_entries
     if (n) *n = g->n;
     if (L) *L = int32_t(g->L);
     if (directed_entries) *directed_entries = int64_t(g->D);
-    if (device_bytes) *device_bytes = g->device_bytes;
+    if (device_bytes) *device_bytes = g->device_bytes + g->cmp_bytes;
     return HG_OK;
 }
 

@@ -95,6 +95,8 @@ struct Inst {
     uint32_t flags;   // kFlagAggregate: combine decrements in LDS
     uint32_t heavy;   // cascade removals with more entries are deferred to the next step
     uint32_t defer;   // entries per deferred chunk (>= kMinDefer)
+    uint32_t tail_heavy;   // the deferral threshold of light steps (tail_chunks chunks or fewer in all)
+    uint32_t tail_chunks;
 };
 
 __device__ __forceinline__ uint32_t* rec_of(const Inst& in, uint32_t v) { return in.deg + uint64_t(v) * in.L; }
@@ -105,6 +107,9 @@ __device__ __forceinline__ void clear_alive(const Inst& in, uint32_t v) {
     atomicAnd(&in.alive[v >> 5], ~(1u << (v & 31)));
 }
 constexpr uint32_t kFlagAggregate = 1;
+// The instance's rows are a compacted copy (k_cmp_*): every target was alive
+// when the copy was made, so the z pre-filter (adjz) does not apply.
+constexpr uint32_t kFlagCompact = 2;
 
 // Per-instance state; the atomically updated words sit on lines of their own.
 // Step s uses the slots of parity s & 1: its phase and level (written once,
@@ -127,7 +132,7 @@ struct alignas(128) Control {
     int32_t pad[30];
 };
 
-enum Err : int32_t { kErrNone = 0, kErrDefer = 2 };
+enum Err : int32_t { kErrNone = 0, kErrDefer = 2, kErrCompact = 3 };
 
 // ----------------------------------------------------------------- helpers
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
@@ -835,6 +840,7 @@ struct CascShared {
     uint32_t tail;
     uint32_t removed;
     uint32_t used;  // distinct keys in the table
+    uint32_t heavy;  // this step's deferral threshold
 };
 
 // Places the vertices removed by this wavefront's lanes (rem) into the
@@ -850,7 +856,7 @@ __device__ __forceinline__ void emit_removed(const Inst& in, InstState* st, Cont
     const uint32_t cap = in.cap;
     uint64_t d = 0;
     if (rem) d = uint64_t(in.off[u + 1] - in.off[u]);
-    const bool light = rem && d <= in.heavy;
+    const bool light = rem && d <= sh->heavy;
     const uint64_t lm = __ballot(light);
     bool placed = false;
     if (lm) {
@@ -1000,7 +1006,8 @@ __device__ __forceinline__ void cascade_expand(const Inst& in, InstState* st, Co
                 }
                 const uint64_t p = sh->start[lo] + (j - sh->pre[lo]);
                 e[r] = in.adj[p];
-                live[r] = in.adjz[p] >= lmd;  // z(target) < lmd: never alive in this instance
+                // z(target) < lmd: never alive in this instance
+                live[r] = (in.flags & kFlagCompact) != 0 || in.adjz[p] >= lmd;
             }
         }
 #pragma unroll
@@ -1048,6 +1055,16 @@ __global__ void __launch_bounds__(kBlock) k_cascade(const Inst* insts, InstState
         sh.phase[threadIdx.x] = st->dphase[par];
         sh.level[threadIdx.x] = st->dk[par];
         sh.chunks[threadIdx.x] = st->pdefer[par][0] + st->pfront[par][0];  // one of them is zero
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        // A light step (few chunks in all, most blocks idle: the tail) keeps
+        // heavier removals in the discovering block's ring rather than paying
+        // a further step for them.
+        unsigned long long tot = 0;
+        for (uint32_t i = 0; i < I; ++i)
+            if (sh.phase[i] != kPhaseDone) tot += sh.chunks[i];
+        sh.heavy = tot <= insts[0].tail_chunks ? insts[0].tail_heavy : insts[0].heavy;
     }
     __syncthreads();
     uint64_t seq = 0;  // chunks of the instances before i
@@ -1127,6 +1144,129 @@ __global__ void __launch_bounds__(kBlock) k_idx_hist(const int32_t* idx, uint32_
     __syncthreads();
     for (uint32_t t = threadIdx.x; t < lds_k; t += blockDim.x)
         if (s_h[t]) atomicAdd(reinterpret_cast<unsigned long long*>(&hist[uint64_t(i) * kcap + t]), (unsigned long long)s_h[t]);
+}
+
+// ------------------------------------------------------- row compaction
+// Between steps, when an instance has no deferred chunks pending (a level
+// boundary), its rows can be rebuilt with only the entries whose target is
+// still alive.  Late in a decomposition the surviving vertices are hubs
+// whose rows are mostly dead neighbours (on the 100M-vertex workload about
+// 90 % of the entries the tail visits), and every removal walks its whole
+// row.  At a level boundary deg[v][l] of an alive v is exactly its number
+// of alive neighbours in layer l (each removed vertex decremented it once),
+// so the new row length is known in advance: off' = exclusive prefix sum of
+// sum_l deg[v][l] over alive v.  The rows are copied in chunks of
+// kCmpChunk entries, one wavefront per chunk, each reserving its places in
+// the vertex's new row with one atomic on a per-vertex cursor: the order of
+// a row's entries changes, which no kernel depends on.  The copy is then
+// checked against the degrees (every cursor must end at off'[v+1]).
+constexpr uint32_t kCmpChunk = 256;
+
+// newlen[v] = sum_l deg[v][l] and nch[v] = chunks of its current row for
+// alive v, 0 otherwise; index n gets 0 (the prefix sums run over n + 1
+// items); only the sum when !kWrite.  words[0] += the sum of newlen (one
+// atomic per block).
+template <bool kWrite>
+__global__ void __launch_bounds__(kBlock) k_cmp_count(const Inst in, uint64_t* newlen, uint64_t* nch,
+                                                      unsigned long long* words) {
+    __shared__ unsigned long long s_sum[kWavesPerBlock];
+    unsigned long long a = 0;
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t v = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; v <= in.n; v += stride) {
+        uint64_t len = 0, ch = 0;
+        if (v < in.n && is_alive(in, uint32_t(v))) {
+            const uint32_t* d = rec_of(in, uint32_t(v));
+            for (uint32_t l = 0; l < in.L; ++l) len += d[l];
+            ch = (uint64_t(in.off[v + 1] - in.off[v]) + kCmpChunk - 1) / kCmpChunk;
+        }
+        if (kWrite) {
+            newlen[v] = len;
+            nch[v] = ch;
+        }
+        a += len;
+    }
+    a = wave_sum_u64(a);
+    if (lane_id() == 0) s_sum[threadIdx.x / kWave] = a;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int w = 0; w < kWavesPerBlock; ++w) t += s_sum[w];
+        if (t) atomicAdd(&words[0], t);
+    }
+}
+
+// Chunk list (v, c) of the alive rows, and each alive vertex's cursor set to
+// the start of its new row.
+__global__ void k_cmp_items(const Inst in, const int64_t* new_off, const uint64_t* item_off, uint64_t* items,
+                            uint64_t* cursor) {
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t v = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; v < in.n; v += stride) {
+        const uint64_t b = item_off[v], e = item_off[v + 1];
+        for (uint64_t c = b; c < e; ++c) items[c] = v | ((c - b) << 32);
+        cursor[v] = uint64_t(new_off[v]);
+    }
+}
+
+// One wavefront per chunk: the entries whose target is alive go to the
+// vertex's new row at places reserved with one atomic on its cursor; a
+// reservation past the row's end (cannot happen when the degrees are
+// exact) is not written and flags the error.
+__global__ void __launch_bounds__(kBlock) k_cmp_fill(const Inst in, const uint64_t* items, uint64_t nitems,
+                                                     const int64_t* new_off, uint64_t* cursor, uint32_t* new_adj,
+                                                     Control* ctl) {
+    const uint32_t lane = lane_id();
+    const uint64_t wave = (blockIdx.x * uint64_t(blockDim.x) + threadIdx.x) / kWave;
+    const uint64_t nwaves = uint64_t(gridDim.x) * blockDim.x / kWave;
+    constexpr uint32_t P = kCmpChunk / kWave;
+    for (uint64_t it = wave; it < nitems; it += nwaves) {
+        const uint64_t x = items[it];
+        const uint32_t v = uint32_t(x);
+        const uint64_t c = x >> 32;
+        const uint64_t beg = uint64_t(in.off[v]) + c * kCmpChunk, end = uint64_t(in.off[v + 1]);
+        uint32_t e[P];
+        bool keep[P];
+#pragma unroll
+        for (uint32_t q = 0; q < P; ++q) {
+            const uint64_t p = beg + q * kWave + lane;
+            keep[q] = p < end;
+            e[q] = keep[q] ? in.adj[p] : 0u;
+        }
+        uint32_t cntq[P], total = 0;
+        uint64_t mask[P];
+#pragma unroll
+        for (uint32_t q = 0; q < P; ++q) {
+            if (keep[q]) keep[q] = is_alive(in, div_L(e[q], in.L, in.lmagic));
+            mask[q] = __ballot(keep[q]);
+            cntq[q] = uint32_t(__popcll(mask[q]));
+            total += cntq[q];
+        }
+        if (total == 0) continue;
+        unsigned long long base = 0;
+        if (lane == 0) base = atomicAdd(reinterpret_cast<unsigned long long*>(&cursor[v]), (unsigned long long)total);
+        base = __shfl(base, 0, kWave);
+        const uint64_t row_end = uint64_t(new_off[v + 1]);
+        if (base + total > row_end) {
+            if (lane == 0) atomicCAS(&ctl->error, 0, int32_t(kErrCompact));
+            continue;
+        }
+        uint64_t at = base;
+#pragma unroll
+        for (uint32_t q = 0; q < P; ++q) {
+            if (keep[q]) {
+                const uint32_t r = __builtin_amdgcn_mbcnt_hi(uint32_t(mask[q] >> 32),
+                                                             __builtin_amdgcn_mbcnt_lo(uint32_t(mask[q]), 0u));
+                new_adj[at + r] = e[q];
+            }
+            at += cntq[q];
+        }
+    }
+}
+
+// Every alive vertex's cursor must have reached the end of its new row.
+__global__ void k_cmp_check(const Inst in, const int64_t* new_off, const uint64_t* cursor, Control* ctl) {
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t v = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; v < in.n; v += stride)
+        if (is_alive(in, uint32_t(v)) && cursor[v] != uint64_t(new_off[v + 1])) atomicCAS(&ctl->error, 0, int32_t(kErrCompact));
 }
 
 // ------------------------------------------------------- densest subgraph

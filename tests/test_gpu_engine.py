@@ -273,3 +273,25 @@ def test_phase_timing():
         eng.set_phase_timing(False)
         eng.decompose()
         assert eng.timing().scan_ms == -1
+
+
+@pytest.mark.parametrize("n,L,m,seed,gamma", [(3000, 5, 12000, 31, 2.3), (20000, 8, 90000, 32, 2.1),
+                                              (60000, 4, 400000, 33, 2.2), (500, 2, 20000, 34, 2.5)])
+def test_row_compaction(monkeypatch, n, L, m, seed, gamma):
+    """HG_COMPACT=2 compacts rows between levels on every graph (off by
+    default): exact against the oracle, also on a second decomposition of the
+    same handle (which starts from the full rows again) and with a small grid
+    (many chunks per block)."""
+    monkeypatch.setenv("HG_COMPACT", "2")
+    mg = MultilayerGraph.generate(small_spec(n, L, m, seed, gamma=gamma))
+    ref = O.coreindex(mg.n, mg.layers())
+    for blocks in (0, 7):
+        if blocks:
+            monkeypatch.setenv("HG_BLOCKS", str(blocks))
+        say(f"case compaction n={n} L={L} m={m} seed={seed} blocks={blocks}")
+        with Engine.from_graph(mg) as eng:
+            for rep in range(2):
+                eng.decompose()
+                assert np.array_equal(eng.core_index(), ref), f"repeat {rep}"
+                cl, kmax = eng.cells()
+                assert np.array_equal(kmax, ref.max(axis=1))
