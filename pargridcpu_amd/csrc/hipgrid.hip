@@ -717,6 +717,7 @@ int decompose_impl(hg_graph* g) {
         // step 0 (level-0 init and its cascade) is init_ms, as without
         // phase timing; scans and cascades of steps 1..s-1 are summed
         double scan = 0, cas = 0;
+        const bool plog = env_int("HG_PHASE_LOG", 0) == 1;  // diagnostic: every step's two phases
         for (uint32_t t = 0; t < s; ++t) {
             float a = 0, b = 0;
             HIP_TRY(hipEventElapsedTime(&a, g->pev[2 * size_t(t)], g->pev[2 * size_t(t) + 1]));
@@ -727,6 +728,7 @@ int decompose_impl(hg_graph* g) {
                 scan += a;
                 cas += b;
             }
+            if (plog) std::fprintf(stderr, "[hg] step %u scan %.4f cascade %.4f\n", t, a, b);
         }
         g->timing.init_ms = ms_init;
         g->timing.scan_ms = scan;
