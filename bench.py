@@ -54,46 +54,95 @@ def measured_traffic(name: str, key: str = "traffic_bytes_per_decomposition"):
     return None, None
 
 
-def cpu_baseline(spec, kind: str, budget_s: float = 30.0):
-    """Times the reference's ParGrid (built from source into oracle/_ref,
-    -O3) or the oracle's single-thread ParGrid restatement on a bounded
-    sample of the workload: the same generator at reduced n with the same
-    per-vertex edge density, sized to ~10-30 s of CPU work."""
-    from pargridcpu_amd.graph import MultilayerGraph
+def host_info() -> dict:
+    """The host the CPU baseline runs on: hardware threads and CPU model."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "cpu_model": model}
+
+
+def run_reference_pargrid(g, threads: int) -> dict:
+    """Runs the reference's own driver (oracle/_ref/firmcore_ref, built from
+    its sources at -O3) as `-m ParGrid -num_thread threads` on g written in
+    the reference's text format; returns its printed elapsed time (its own
+    omp_get_wtime bracket around FCSyncLeft::Execute, main.cpp:229-240) and
+    peak RSS."""
+    ref_bin = os.path.join(ROOT, "oracle", "_ref", "firmcore_ref")
+    tmp = tempfile.mkdtemp(prefix="hgcpu_")
+    try:
+        ds = os.path.join(tmp, "dataset", "bench")
+        os.makedirs(ds)
+        os.makedirs(os.path.join(tmp, "run"))
+        g.write_text(ds)
+        out = subprocess.run([ref_bin, "-d", "bench", "-m", "ParGrid", "-num_thread", str(threads)],
+                             cwd=os.path.join(tmp, "run"), capture_output=True, text=True, timeout=900)
+        secs = rss = None
+        for line in out.stdout.splitlines():
+            if "Path left Core Parallel Sync Elapsed time" in line:
+                secs = float(line.split(":")[1].split()[0])
+            if line.startswith("mem = "):
+                rss = float(line.split()[2])
+        if secs is None:
+            raise RuntimeError("reference ParGrid printed no elapsed time:\n" + out.stdout[-2000:] + out.stderr)
+        return {"seconds": secs, "peak_rss_mb": rss}
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+def gpu_time(g, reps: int = 5) -> float:
+    """Median ms of hg_decompose on g (graph resident, after a warm-up)."""
+    from pargridcpu_amd.hipgrid import Engine
+
+    with Engine.from_graph(g) as eng:
+        eng.decompose()
+        ms = []
+        for _ in range(reps):
+            eng.decompose()
+            ms.append(eng.timing().decompose_ms)
+    return sorted(ms)[len(ms) // 2]
+
+
+def cpu_baseline(spec, kind: str, kmax_sum: int | None = None, full_entries: int | None = None) -> dict:
+    """The reference's ParGrid (built from source into oracle/_ref, -O3) or,
+    without it, the oracle's single-thread ParGrid restatement, timed on this
+    host:
+      * value: on a bounded sample of the workload (the same generator at
+        1/100 of n, edges and degree cap, ~10-30 s of CPU work), with the GPU
+        timed on the same sample beside it (gpu_same_sample);
+      * c2_full: the 1M-vertex, 10-layer workload at full size, reference and
+        GPU on the same graph;
+      * full_size: the workload itself -- run when it fits the host, else
+        reported as not run with the reference's memory estimate.
+    cores = host threads the reference ran with (-num_thread = nproc); its
+    peel team is fixed at 32 threads whatever -num_thread says
+    (FCSyncLeft.cpp:51)."""
+    from pargridcpu_amd.graph import C2, MultilayerGraph
 
     ref_bin = os.path.join(ROOT, "oracle", "_ref", "firmcore_ref")
     if kind == "auto":
         kind = "reference" if os.path.exists(ref_bin) else "port"
+    host = host_info()
+    threads = int(host["nproc"] or 1)
     scale = min(1.0, 1_000_000 / spec.n)
     sample = dataclasses.replace(spec, n=max(1000, int(spec.n * scale)), m_lo=max(1, int(spec.m_lo * scale)),
                                  m_hi=max(1, int(spec.m_hi * scale)),
                                  cap_deg=spec.cap_deg * scale if spec.cap_deg else 0.0)
     g = MultilayerGraph.generate(sample)
     D = g.directed_entries()
-    desc = (f"{sample.name if scale == 1.0 else 'sample of ' + spec.name}: n={sample.n} L={sample.L} D={D} "
-            f"(generator at n-scale {scale:g}, same gamma/rho/seed)")
-    threads = os.cpu_count() or 1
+    desc = (f"{'sample of ' + spec.name if scale < 1.0 else spec.name}: n={sample.n} L={sample.L} D={D} (the same "
+            f"generator at 1/{round(1 / scale)} of n, edges and degree cap; same gamma, reshuffle and seed)")
+    rss = None
     if kind == "reference":
-        tmp = tempfile.mkdtemp(prefix="hgcpu_")
-        try:
-            ds = os.path.join(tmp, "dataset", "bench")
-            os.makedirs(ds)
-            os.makedirs(os.path.join(tmp, "run"))
-            g.write_text(ds)
-            threads = int(os.environ.get("OMP_NUM_THREADS", threads))
-            out = subprocess.run([ref_bin, "-d", "bench", "-m", "ParGrid", "-num_thread", str(threads)],
-                                 cwd=os.path.join(tmp, "run"), capture_output=True, text=True, timeout=600)
-            secs = None
-            for line in out.stdout.splitlines():
-                if "Path left Core Parallel Sync Elapsed time" in line:
-                    secs = float(line.split(":")[1].split()[0])
-            if secs is None:
-                raise RuntimeError("reference ParGrid printed no elapsed time:\n" + out.stdout[-2000:] + out.stderr)
-            desc += (f"; reference ParGrid (main.cpp -m ParGrid, -O3), -num_thread {threads}, peel team fixed at 32 "
-                     f"threads (FCSyncLeft.cpp:51), timed by its own omp_get_wtime bracket")
-            cores = 32  # the peel team is fixed at 32 threads whatever -num_thread says
-        finally:
-            shutil.rmtree(tmp, ignore_errors=True)
+        r = run_reference_pargrid(g, threads)
+        secs, rss = r["seconds"], r["peak_rss_mb"]
+        desc += (f"; reference ParGrid (main.cpp -m ParGrid, built at -O3; the reference's default build is -O0) "
+                 f"with -num_thread {threads}, timed by its own omp_get_wtime bracket")
     else:
         from oracle import oracle as O
 
@@ -101,10 +150,47 @@ def cpu_baseline(spec, kind: str, budget_s: float = 30.0):
         O.pargrid(g.n, g.layers())
         secs = time.perf_counter() - t0
         desc += "; oracle ParGrid restatement, single thread"
-        cores = 1
+        threads = 1
     gteps = g.L * D / secs / 1e9
-    return {"value": round(gteps, 6), "unit": "GTEPS", "cores": cores, "kind": kind, "sample": desc,
-            "seconds": round(secs, 4)}
+    gms = gpu_time(g)
+    out = {"value": round(gteps, 6), "unit": "GTEPS", "cores": threads, "kind": kind, "sample": desc,
+           "seconds": round(secs, 4), "peak_rss_mb": rss, "nproc": host["nproc"], "cpu_model": host["cpu_model"],
+           "num_thread": threads, "peel_team": 32 if kind == "reference" else 1,
+           "gpu_same_sample": {"ms": round(gms, 3), "gteps": round(g.L * D / (gms * 1e-3) / 1e9, 3),
+                               "speedup": round(secs * 1e3 / gms, 1)}}
+    del g
+    if kind == "reference":
+        g2 = MultilayerGraph.generate(C2)
+        D2 = g2.directed_entries()
+        r2 = run_reference_pargrid(g2, threads)
+        g2ms = gpu_time(g2)
+        out["c2_full"] = {"workload": C2.name, "n": g2.n, "L": g2.L, "directed_entries": D2,
+                          "reference_seconds": r2["seconds"], "reference_peak_rss_mb": r2["peak_rss_mb"],
+                          "reference_gteps": round(g2.L * D2 / r2["seconds"] / 1e9, 4), "gpu_ms": round(g2ms, 3),
+                          "gpu_gteps": round(g2.L * D2 / (g2ms * 1e-3) / 1e9, 3),
+                          "speedup": round(r2["seconds"] * 1e3 / g2ms, 1)}
+        del g2
+    if scale < 1.0:
+        # The reference's ParGrid memory on the full workload (FCSyncLeft.cpp):
+        # adjacency (4 B per entry + n row pointers and counts per layer,
+        # Graph.cpp:17-50), degs (n rows of L ints plus pointers), a deep
+        # degs + valid copy per lambda-spine node (serial = 1, :134-152) and
+        # an n-byte valid copy per fcgrid cell that is never freed (:10-38).
+        n, L = spec.n, spec.L
+        cells = kmax_sum if kmax_sum is not None else 0
+        ent = full_entries if full_entries is not None else (spec.m_lo + spec.m_hi) * L
+        est = (4 * ent + 12 * n * L) + (24 + 4 * L) * n + L * (25 + 4 * L) * n + n * cells
+        try:
+            mem = os.sysconf("SC_PAGE_SIZE") * os.sysconf("SC_PHYS_PAGES")
+        except (ValueError, OSError):
+            mem = None
+        out["full_size"] = {"workload": spec.name, "status": "not run",
+                            "reason": (f"reference ParGrid needs about {est / 1e9:.0f} GB of host memory here "
+                                       f"({cells} fcgrid cells x {n / 1e6:.0f} MB of never-freed valid copies plus "
+                                       f"adjacency, degs and lambda-spine copies); host memory "
+                                       f"{(mem or 0) / 1e9:.0f} GB"),
+                            "estimated_bytes": int(est), "host_memory_bytes": mem}
+    return out
 
 
 def run_batched(steps: int, count: int = 256, params: str = "tune") -> dict:
@@ -288,19 +374,23 @@ def run_decompose(args, spec) -> dict:
     dev_ms = sum(per_call) / len(per_call)
     achieved = balg / (dev_ms * 1e-3) / 1e9
     traffic, traffic_src = measured_traffic(f"hg_decompose_{args.workload}")
+    l2_hits, _ = measured_traffic(f"hg_decompose_{args.workload}", "l2_hits_per_decomposition")
+    l2_miss, _ = measured_traffic(f"hg_decompose_{args.workload}", "l2_misses_per_decomposition")
     roof = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic, "traffic_source": traffic_src,
+            "counted_hbm_gbs": round(traffic / (dev_ms * 1e-3) / 1e9, 1) if traffic else None,
+            "l2_hit_rate": round(l2_hits / (l2_hits + l2_miss), 4) if l2_hits is not None and l2_miss else None,
             "kernel": "hg_decompose (k_init + per-level k_scan/k_cascade)", "algorithmic_bytes": balg,
             "avg_ms": round(dev_ms, 4)}
+    eng.close()
+    del eng, g
     cpu = None
     if args.cpu_baseline != "off":
         try:
-            cpu = cpu_baseline(spec, args.cpu_baseline)
+            cpu = cpu_baseline(spec, args.cpu_baseline, kmax_sum=int(kmax.sum()), full_entries=D)
             log(f"[bench] cpu baseline: {cpu}")
         except Exception as e:  # reported, never fatal for the GPU number
             cpu = {"value": None, "unit": "GTEPS", "cores": None, "kind": args.cpu_baseline, "sample": f"failed: {e}"}
-    eng.close()
-    del eng, g
     return {"value": round(gteps, 4), "ms_per_step": round(ms, 4), "roofline": roof, "cpu_baseline": cpu,
             "config": {"workload": spec.name, "n": n, "L": L, "directed_entries": D, "gamma": spec.gamma,
                        "rho": spec.rho, "offsets": "int64", "parallelism": "single-gpu"}}

@@ -164,3 +164,19 @@ def test_committed_profiles_carry_traffic():
         assert val is not None and val > 10**10
         text = open(os.path.join(root, src)).read()
         assert "rocprofv3 --kernel-trace --stats" in text and "--pmc FETCH_SIZE" in text and "--pmc WRITE_SIZE" in text
+
+
+def test_committed_c3_profile_carries_l2_counts():
+    """The bench line's L2 hit rate comes from the TCC_HIT/TCC_MISS run in
+    the committed summary of the 100M-vertex workload."""
+    import importlib.util
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    spec = importlib.util.spec_from_file_location("bench_mod", os.path.join(root, "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    hits, src = bench.measured_traffic("hg_decompose_c3", "l2_hits_per_decomposition")
+    miss, _ = bench.measured_traffic("hg_decompose_c3", "l2_misses_per_decomposition")
+    assert src == os.path.join("profiles", "hg_decompose_c3.md")
+    assert hits > 10**9 and miss > 10**9
+    assert "--pmc TCC_HIT_sum TCC_MISS_sum" in open(os.path.join(root, src)).read()
