@@ -47,6 +47,7 @@ extern "C" {
 #define HG_ERR_RANGE (-4)   /* size limit exceeded (see hg_graph_create) */
 #define HG_ERR_STATE (-5)   /* call out of order (e.g. results before decompose) */
 #define HG_ERR_DEVICE (-6)  /* device-side check failed (watchdog, overflow) */
+#define HG_ERR_CANCELLED (-7) /* the request was cancelled before it started */
 
 /* One layer as symmetric CSR: row_ptr has n+1 entries (int32 or int64 as
  * chosen by offset_bits), col[row_ptr[v] .. row_ptr[v+1]) lists the
@@ -269,13 +270,17 @@ int hg_batch_destroy(hg_batch* b);
 typedef struct hg_server hg_server;
 
 typedef struct hg_server_config {
-    int64_t hbm_budget;     /* device bytes the loop may hold at once */
+    int64_t hbm_budget;     /* device bytes the loop may hold at once (registered graphs included) */
     int64_t large_entries;  /* requests above this many directed entries run alone */
     int32_t lanes;          /* concurrent workers (streams), 1..16 */
     int32_t max_batch;      /* requests per batch, >= 1 */
     int32_t plain;          /* 1: one request at a time on the single-graph engine */
     int32_t min_batch;      /* small requests waiting before a lane batches them (0: 8) */
     hg_batch_params params; /* launch shape of the batches (block == 0: defaults) */
+    int32_t reuse_results;  /* 1: a registered graph's resident decomposition answers repeat
+                               decompositions and densest searches on it; 0: every request
+                               decomposes again */
+    int32_t reserved;
 } hg_server_config;
 
 #define HG_REQ_DECOMPOSE 0
@@ -297,19 +302,43 @@ typedef struct hg_server_summary {
     int64_t submitted, completed, failed, batches;
     double p50_ms, p99_ms, mean_ms, max_ms; /* latency of completed requests */
     double span_s;                          /* first submit -> last completion */
-    int64_t peak_bytes;                     /* most device bytes admitted at once */
+    int64_t peak_bytes;                     /* most device bytes admitted at once (registered graphs included) */
+    int64_t cancelled;                      /* requests cancelled before they started */
+    int64_t resident_graphs, resident_bytes; /* registered graphs and the device bytes they hold */
 } hg_server_summary;
 
 int hg_server_create(const hg_server_config* cfg, hg_server** out);
 /* Queues a request (the graph is copied); *id names it. */
 int hg_server_submit(hg_server* s, const hg_batch_graph* graph, int32_t kind, double beta, int64_t* id);
+/* Registers a graph: it is uploaded and laid out once (by the single-graph
+ * engine, on a stream of its own) and stays resident in device memory, its
+ * bytes counted against hbm_budget, until hg_server_unregister.  Fails with
+ * HG_ERR_NOMEM when the registered graphs would exceed the budget.  The
+ * caller's arrays may be reused on return. */
+int hg_server_register(hg_server* s, const hg_batch_graph* graph, int64_t* gid);
+/* Queues a request on a registered graph: nothing is copied or uploaded.
+ * Requests on one graph run one at a time on its resident handle; with
+ * reuse_results the decomposition the handle holds answers repeat
+ * decompositions, and densest searches only score its cells. */
+int hg_server_submit_resident(hg_server* s, int64_t gid, int32_t kind, double beta, int64_t* id);
+/* Frees a registered graph; HG_ERR_STATE while requests on it are queued or
+ * running. */
+int hg_server_unregister(hg_server* s, int64_t gid);
+/* Cancels a request that has not started: it is done with status
+ * HG_ERR_CANCELLED (hg_server_wait returns that code).  A started request
+ * runs to the end (HG_ERR_STATE); cancelling a cancelled one is a no-op. */
+int hg_server_cancel(hg_server* s, int64_t id);
+/* Test hook: the after-th device allocation from now on (1: the next) fails
+ * as if the device were full, in any handle of this process; 0 disarms it.
+ * Lets tests show that a failed allocation fails only its own request. */
+int hg_debug_fail_alloc(int64_t after);
 /* Blocks until request id is done; fills info (may be NULL). */
 int hg_server_wait(hg_server* s, int64_t id, hg_request_info* info);
 /* Results of a done request: core index [L][n] and kmax [L] (either may be NULL). */
 int hg_server_result(hg_server* s, int64_t id, int32_t* core_index, int32_t* kmax);
 /* Forgets a done request and its results. */
 int hg_server_release(hg_server* s, int64_t id);
-/* Blocks until every submitted request is done. */
+/* Blocks until every submitted request is done (or cancelled). */
 int hg_server_drain(hg_server* s);
 int hg_server_stats(hg_server* s, hg_server_summary* out);
 /* Finishes the queued work, stops the lanes and frees the server. */

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstddef>
 #include <cstdlib>
 #include <cstring>
@@ -92,8 +93,17 @@ inline hipError_t malloc_or_trim(void** p, size_t bytes) {
     return e;
 }
 
+// Test hook (hg_debug_fail_alloc): while positive, counts device allocations
+// down; the one that takes it to zero fails as if the device were full.
+inline std::atomic<int64_t>& fail_countdown() {
+    static std::atomic<int64_t> c{0};
+    return c;
+}
+
 inline hipError_t pool_malloc(void** p, size_t bytes) {
     *p = nullptr;
+    if (fail_countdown().load(std::memory_order_relaxed) > 0 && fail_countdown().fetch_sub(1) == 1)
+        return hipErrorOutOfMemory;
     DevPool& d = dev_pool();
     if (d.cap == 0) return malloc_or_trim(p, bytes);  // cache off: exact sizes, freed at once
     size_t c = 256;

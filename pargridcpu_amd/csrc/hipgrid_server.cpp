@@ -16,6 +16,15 @@
 // overlap on the device.  A request is admitted only while the bytes of
 // everything in flight stay within `hbm_budget`.
 //
+// Graphs can also be registered once (hg_server_register): uploaded and laid
+// out by the single-graph engine at registration and kept resident in
+// device memory, their bytes counted against the budget until unregistered.
+// Requests on a registered graph (hg_server_submit_resident) copy and upload
+// nothing; they run on its resident handle, one at a time per graph, and,
+// with reuse_results, a decomposition already on the device answers repeat
+// decompositions and every densest search (which then only scores cells).
+// A request that has not started can be cancelled.
+//
 // plain = 1 is the baseline: one lane, one request at a time, each on the
 // single-graph engine.
 //
@@ -27,6 +36,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -45,8 +55,11 @@ using Clock = std::chrono::steady_clock;
 
 namespace {
 
+struct Resident;
+
 struct Request {
     int64_t id = 0;
+    Resident* res = nullptr;  // a registered graph, or nullptr (the graph is copied in)
     int32_t kind = HG_REQ_DECOMPOSE;
     double beta = 1.0;
     int64_t n = 0;
@@ -65,6 +78,23 @@ struct Request {
     hg_wds wds{};
     int32_t batch_size = 0;
     Clock::time_point t_submit, t_start, t_done;
+};
+
+// A registered graph: its single-graph engine handle on a stream of its own,
+// and the host copy of its last decomposition.
+struct Resident {
+    int64_t gid = 0;
+    hg_graph* g = nullptr;
+    hipStream_t stream = nullptr;
+    int64_t n = 0;
+    int32_t L = 0;
+    uint64_t D = 0;
+    int64_t bytes = 0;
+    std::mutex run;           // one request at a time on the handle
+    bool decomposed = false;  // the handle holds a decomposition (guarded by run)
+    std::unique_ptr<int32_t[]> ci;
+    std::vector<int32_t> kmax;
+    int64_t pending = 0;  // requests queued or running on it (guarded by the server's mutex)
 };
 
 double ms_between(Clock::time_point a, Clock::time_point b) {
@@ -103,9 +133,10 @@ struct hg_server {
     std::condition_variable cv_work, cv_done;
     std::deque<Request*> pending;
     std::unordered_map<int64_t, std::unique_ptr<Request>> reqs;
-    int64_t next_id = 1;
-    int64_t inflight = 0, peak = 0;
-    int64_t submitted = 0, completed = 0, failed = 0, batches = 0;
+    std::unordered_map<int64_t, std::unique_ptr<Resident>> residents;
+    int64_t next_id = 1, next_gid = 1;
+    int64_t inflight = 0, peak = 0, resident_bytes = 0;
+    int64_t submitted = 0, completed = 0, failed = 0, cancelled = 0, batches = 0;
     std::vector<double> latencies;
     bool have_first = false;
     Clock::time_point first_submit, last_done;
@@ -115,13 +146,18 @@ struct hg_server {
 
 namespace {
 
-bool is_large(const hg_server* s, const Request* r) { return s->cfg.plain || int64_t(r->D) > s->cfg.large_entries; }
+// Requests that run alone: large ones, every one in plain mode, and those on
+// registered graphs (their resident handle).
+bool is_large(const hg_server* s, const Request* r) {
+    return s->cfg.plain || r->res || int64_t(r->D) > s->cfg.large_entries;
+}
 
 void finish(hg_server* s, std::vector<Request*>& rs, int rc, const std::string& err, int64_t bytes) {
     const auto now = Clock::now();
     std::lock_guard<std::mutex> lk(s->mu);
     s->inflight -= bytes;
     for (Request* r : rs) {
+        if (r->res) --r->res->pending;
         r->t_done = now;
         r->status = rc == HG_OK ? 0 : rc;
         r->err = err;
@@ -157,6 +193,34 @@ int run_alone(Request* r, hipStream_t stream) {
     }
     if (!rc && r->kind == HG_REQ_DENSEST) rc = hg_densest(g, r->beta, &r->wds);
     hg_graph_destroy(g);
+    return rc;
+}
+
+// A request on a registered graph: decompose on its resident handle unless
+// the handle already holds the decomposition and results may be reused,
+// then copy the results (and score the cells for a densest search).
+int run_resident(hg_server* s, Request* r) {
+    Resident* res = r->res;
+    std::lock_guard<std::mutex> lk(res->run);
+    int rc = HG_OK;
+    if (!res->decomposed || !s->cfg.reuse_results) {
+        res->decomposed = false;
+        rc = hg_decompose(res->g);
+        if (!rc) {
+            if (!res->ci) res->ci.reset(new int32_t[size_t(res->n) * size_t(res->L)]);
+            rc = hg_core_index(res->g, res->ci.get());
+        }
+        if (!rc) {
+            res->kmax.resize(size_t(res->L));
+            rc = hg_cells(res->g, nullptr, 0, res->kmax.data());
+        }
+        if (rc) return rc;
+        res->decomposed = true;
+    }
+    r->core_index.reset(new int32_t[size_t(res->n) * size_t(res->L)]);
+    hgc::par_memcpy(r->core_index.get(), res->ci.get(), 4 * size_t(res->n) * size_t(res->L));
+    r->kmax = res->kmax;
+    if (r->kind == HG_REQ_DENSEST) rc = hg_densest(res->g, r->beta, &r->wds);
     return rc;
 }
 
@@ -222,16 +286,16 @@ void lane_main(hg_server* s) {
             for (;;) {
                 if (!s->pending.empty()) {
                     Request* r = s->pending.front();
-                    if (r->est_bytes > s->cfg.hbm_budget) {  // can never run: fail it
+                    if (r->est_bytes + s->resident_bytes > s->cfg.hbm_budget) {  // cannot run: fail it
                         s->pending.pop_front();
                         r->status = HG_ERR_NOMEM;
-                        r->err = "request exceeds the device-memory budget";
+                        r->err = "request exceeds the device-memory budget (registered graphs included)";
                         r->t_done = Clock::now();
                         ++s->failed;
                         s->cv_done.notify_all();
                         continue;
                     }
-                    if (s->inflight + r->est_bytes <= s->cfg.hbm_budget) break;
+                    if (s->inflight + s->resident_bytes + r->est_bytes <= s->cfg.hbm_budget) break;
                 }
                 if (s->stop && s->pending.empty()) {
                     lk.unlock();
@@ -253,7 +317,7 @@ void lane_main(hg_server* s) {
             while (!s->pending.empty() && int32_t(take.size()) < cap) {
                 Request* r = s->pending.front();
                 if (!take.empty() && is_large(s, r)) break;  // a large one runs alone, in arrival order
-                if (s->inflight + bytes + r->est_bytes > s->cfg.hbm_budget) break;  // est covers both engines
+                if (s->inflight + s->resident_bytes + bytes + r->est_bytes > s->cfg.hbm_budget) break;
                 s->pending.pop_front();
                 take.push_back(r);
                 bytes += r->est_bytes;
@@ -268,7 +332,8 @@ void lane_main(hg_server* s) {
         }
         int rc;
         try {
-            rc = large || alone ? run_alone(take[0], stream) : run_batch(s, take, stream);
+            rc = take[0]->res ? run_resident(s, take[0])
+                 : large || alone ? run_alone(take[0], stream) : run_batch(s, take, stream);
         } catch (const std::bad_alloc&) {
             rc = fail(HG_ERR_NOMEM, "host allocation failed");
         }
@@ -398,6 +463,132 @@ int hg_server_submit(hg_server* s, const hg_batch_graph* graph, int32_t kind, do
     return HG_OK;
 }
 
+int hg_server_register(hg_server* s, const hg_batch_graph* graph, int64_t* gid) {
+    if (!s || !graph || !gid) return fail(HG_ERR_ARG, "NULL argument");
+    if (!graph->layers) return fail(HG_ERR_ARG, "layers is NULL");
+    std::unique_ptr<Resident> res(new (std::nothrow) Resident());
+    if (!res) return fail(HG_ERR_NOMEM, "host allocation failed");
+    HIP_TRY(hipSetDevice(s->device));
+    HIP_TRY(hipStreamCreateWithFlags(&res->stream, hipStreamNonBlocking));
+    int rc = hg_graph_create(graph->layers, graph->L, graph->n, graph->offset_bits, res->stream, &res->g);
+    int64_t D = 0, bytes = 0;
+    if (!rc) rc = hg_graph_info(res->g, nullptr, nullptr, &D, &bytes);
+    if (rc) {
+        const std::string err = hg_last_error();
+        if (res->g) hg_graph_destroy(res->g);
+        (void)hipStreamDestroy(res->stream);
+        return fail(rc, err);
+    }
+    res->n = graph->n;
+    res->L = graph->L;
+    res->D = uint64_t(D);
+    // a densest search adds the edge histograms, L * (largest degree + 2)
+    // words, kept on the handle
+    uint64_t maxdeg = 0;
+    for (int l = 0; l < graph->L; ++l)
+        for (int64_t v = 0; v < graph->n; ++v) {
+            const int64_t d = graph->offset_bits == 64
+                                  ? static_cast<const int64_t*>(graph->layers[l].row_ptr)[v + 1] -
+                                        static_cast<const int64_t*>(graph->layers[l].row_ptr)[v]
+                                  : int64_t(static_cast<const int32_t*>(graph->layers[l].row_ptr)[v + 1]) -
+                                        static_cast<const int32_t*>(graph->layers[l].row_ptr)[v];
+            maxdeg = std::max<uint64_t>(maxdeg, uint64_t(std::max<int64_t>(d, 0)));
+        }
+    res->bytes = bytes + int64_t(8 * uint64_t(graph->L) * (maxdeg + 2));
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->resident_bytes + res->bytes > s->cfg.hbm_budget) {
+        hg_graph_destroy(res->g);
+        (void)hipStreamDestroy(res->stream);
+        return fail(HG_ERR_NOMEM, "registered graphs would exceed the device-memory budget");
+    }
+    res->gid = s->next_gid++;
+    *gid = res->gid;
+    s->resident_bytes += res->bytes;
+    s->peak = std::max(s->peak, s->inflight + s->resident_bytes);
+    s->residents.emplace(res->gid, std::move(res));
+    return HG_OK;
+}
+
+int hg_server_submit_resident(hg_server* s, int64_t gid, int32_t kind, double beta, int64_t* id) {
+    if (!s || !id) return fail(HG_ERR_ARG, "NULL argument");
+    if (kind != HG_REQ_DECOMPOSE && kind != HG_REQ_DENSEST) return fail(HG_ERR_ARG, "unknown request kind");
+    if (kind == HG_REQ_DENSEST && !std::isfinite(beta)) return fail(HG_ERR_ARG, "beta must be finite");
+    std::unique_ptr<Request> r(new (std::nothrow) Request());
+    if (!r) return fail(HG_ERR_NOMEM, "host allocation failed");
+    r->t_submit = Clock::now();
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->residents.find(gid);
+    if (it == s->residents.end()) return fail(HG_ERR_ARG, "unknown graph id");
+    Resident* res = it->second.get();
+    r->res = res;
+    r->kind = kind;
+    r->beta = beta;
+    r->n = res->n;
+    r->L = res->L;
+    r->D = res->D;
+    r->est_bytes = 0;  // the resident bytes are counted already
+    r->id = s->next_id++;
+    *id = r->id;
+    ++res->pending;
+    if (!s->have_first) {
+        s->have_first = true;
+        s->first_submit = r->t_submit;
+    }
+    ++s->submitted;
+    s->pending.push_back(r.get());
+    s->reqs.emplace(r->id, std::move(r));
+    s->cv_work.notify_one();
+    return HG_OK;
+}
+
+int hg_server_unregister(hg_server* s, int64_t gid) {
+    if (!s) return fail(HG_ERR_ARG, "server is NULL");
+    std::unique_ptr<Resident> res;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        auto it = s->residents.find(gid);
+        if (it == s->residents.end()) return fail(HG_ERR_ARG, "unknown graph id");
+        if (it->second->pending > 0) return fail(HG_ERR_STATE, "requests on the graph are pending");
+        res = std::move(it->second);
+        s->residents.erase(it);
+        s->resident_bytes -= res->bytes;
+        s->cv_work.notify_all();
+    }
+    hg_graph_destroy(res->g);
+    (void)hipStreamDestroy(res->stream);
+    return HG_OK;
+}
+
+int hg_server_cancel(hg_server* s, int64_t id) {
+    if (!s) return fail(HG_ERR_ARG, "server is NULL");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) return fail(HG_ERR_ARG, "unknown request id");
+    Request* r = it->second.get();
+    auto q = std::find(s->pending.begin(), s->pending.end(), r);
+    if (q == s->pending.end()) {
+        if (r->status == HG_ERR_CANCELLED) return HG_OK;
+        return fail(HG_ERR_STATE, r->status == 1 ? "request already started" : "request already done");
+    }
+    s->pending.erase(q);
+    if (r->res) --r->res->pending;
+    r->status = HG_ERR_CANCELLED;
+    r->err = "request cancelled";
+    r->t_done = Clock::now();
+    r->rp.clear();
+    r->col.clear();
+    ++s->cancelled;
+    s->cv_done.notify_all();
+    s->cv_work.notify_all();
+    return HG_OK;
+}
+
+int hg_debug_fail_alloc(int64_t after) {
+    if (after < 0) return fail(HG_ERR_ARG, "after must be >= 0");
+    hgc::fail_countdown().store(after);
+    return HG_OK;
+}
+
 int hg_server_wait(hg_server* s, int64_t id, hg_request_info* info) {
     if (!s) return fail(HG_ERR_ARG, "server is NULL");
     std::unique_lock<std::mutex> lk(s->mu);
@@ -447,7 +638,7 @@ int hg_server_release(hg_server* s, int64_t id) {
 int hg_server_drain(hg_server* s) {
     if (!s) return fail(HG_ERR_ARG, "server is NULL");
     std::unique_lock<std::mutex> lk(s->mu);
-    s->cv_done.wait(lk, [&] { return s->completed + s->failed == s->submitted; });
+    s->cv_done.wait(lk, [&] { return s->completed + s->failed + s->cancelled == s->submitted; });
     return HG_OK;
 }
 
@@ -459,6 +650,9 @@ int hg_server_stats(hg_server* s, hg_server_summary* out) {
     out->completed = s->completed;
     out->failed = s->failed;
     out->batches = s->batches;
+    out->cancelled = s->cancelled;
+    out->resident_graphs = int64_t(s->residents.size());
+    out->resident_bytes = s->resident_bytes;
     out->p50_ms = percentile(s->latencies, 0.50);
     out->p99_ms = percentile(s->latencies, 0.99);
     double sum = 0, mx = 0;
@@ -481,6 +675,10 @@ int hg_server_destroy(hg_server* s) {
     }
     s->cv_work.notify_all();
     for (auto& t : s->lanes) t.join();
+    for (auto& kv : s->residents) {
+        hg_graph_destroy(kv.second->g);
+        (void)hipStreamDestroy(kv.second->stream);
+    }
     delete s;
     return HG_OK;
 }

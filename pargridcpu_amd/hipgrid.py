@@ -25,7 +25,7 @@ from ._native import ENGINE_LIB, NativeLibraryMissing, _open
 
 HG_OK = 0
 _ERRNAMES = {-1: "HG_ERR_ARG", -2: "HG_ERR_HIP", -3: "HG_ERR_NOMEM", -4: "HG_ERR_RANGE", -5: "HG_ERR_STATE",
-             -6: "HG_ERR_DEVICE"}
+             -6: "HG_ERR_DEVICE", -7: "HG_ERR_CANCELLED"}
 
 
 class HgError(RuntimeError):
@@ -63,7 +63,8 @@ class BatchParams(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget", C.c_int64), ("large_entries", C.c_int64), ("lanes", C.c_int32), ("max_batch", C.c_int32),
-                ("plain", C.c_int32), ("min_batch", C.c_int32), ("params", BatchParams)]
+                ("plain", C.c_int32), ("min_batch", C.c_int32), ("params", BatchParams), ("reuse_results", C.c_int32),
+                ("reserved", C.c_int32)]
 
 
 class RequestInfo(C.Structure):
@@ -75,7 +76,8 @@ class RequestInfo(C.Structure):
 class ServerSummary(C.Structure):
     _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("failed", C.c_int64), ("batches", C.c_int64),
                 ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("mean_ms", C.c_double), ("max_ms", C.c_double),
-                ("span_s", C.c_double), ("peak_bytes", C.c_int64)]
+                ("span_s", C.c_double), ("peak_bytes", C.c_int64), ("cancelled", C.c_int64),
+                ("resident_graphs", C.c_int64), ("resident_bytes", C.c_int64)]
 
     def as_dict(self) -> dict:
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -120,6 +122,11 @@ SIGNATURES = {
     "hg_batch_destroy": ([_P], C.c_int),
     "hg_server_create": ([C.POINTER(ServerConfig), C.POINTER(_P)], C.c_int),
     "hg_server_submit": ([_P, C.POINTER(BatchGraph), C.c_int32, C.c_double, C.POINTER(C.c_int64)], C.c_int),
+    "hg_server_register": ([_P, C.POINTER(BatchGraph), C.POINTER(C.c_int64)], C.c_int),
+    "hg_server_submit_resident": ([_P, C.c_int64, C.c_int32, C.c_double, C.POINTER(C.c_int64)], C.c_int),
+    "hg_server_unregister": ([_P, C.c_int64], C.c_int),
+    "hg_server_cancel": ([_P, C.c_int64], C.c_int),
+    "hg_debug_fail_alloc": ([C.c_int64], C.c_int),
     "hg_server_wait": ([_P, C.c_int64, C.POINTER(RequestInfo)], C.c_int),
     "hg_server_result": ([_P, C.c_int64, C.POINTER(C.c_int32), C.POINTER(C.c_int32)], C.c_int),
     "hg_server_release": ([_P, C.c_int64], C.c_int),
@@ -434,7 +441,7 @@ class Server:
 
     def __init__(self, hbm_budget: int = 200 << 30, lanes: int = 4, max_batch: int = 64,
                  large_entries: int = 4_000_000, plain: bool = False, params: dict | None = None,
-                 min_batch: int = 0):
+                 min_batch: int = 0, reuse_results: bool = True):
         cfg = ServerConfig()
         cfg.hbm_budget = int(hbm_budget)
         cfg.large_entries = int(large_entries)
@@ -442,6 +449,7 @@ class Server:
         cfg.max_batch = int(max_batch)
         cfg.plain = 1 if plain else 0
         cfg.min_batch = int(min_batch)
+        cfg.reuse_results = 1 if reuse_results else 0
         for k, v in (params or {}).items():
             setattr(cfg.params, k, int(v))
         h = _P()
@@ -455,6 +463,27 @@ class Server:
         _check(lib().hg_server_submit(self._h, C.byref(d), int(kind), float(beta), C.byref(rid)))
         self._shapes[rid.value] = (int(d.n), int(d.L))
         return rid.value
+
+    def register(self, g) -> int:
+        """Uploads g once and keeps it resident; returns its graph id."""
+        d, keep = _graph_desc(g)
+        gid = C.c_int64()
+        _check(lib().hg_server_register(self._h, C.byref(d), C.byref(gid)))
+        self._gshapes = getattr(self, "_gshapes", {})
+        self._gshapes[gid.value] = (int(d.n), int(d.L))
+        return gid.value
+
+    def submit_resident(self, gid: int, kind: int = REQ_DECOMPOSE, beta: float = 1.0) -> int:
+        rid = C.c_int64()
+        _check(lib().hg_server_submit_resident(self._h, int(gid), int(kind), float(beta), C.byref(rid)))
+        self._shapes[rid.value] = self._gshapes[gid]
+        return rid.value
+
+    def unregister(self, gid: int) -> None:
+        _check(lib().hg_server_unregister(self._h, int(gid)))
+
+    def cancel(self, rid: int) -> None:
+        _check(lib().hg_server_cancel(self._h, int(rid)))
 
     def wait(self, rid: int) -> RequestInfo:
         info = RequestInfo()

@@ -13,7 +13,7 @@ import pytest
 
 from oracle import oracle as O
 from pargridcpu_amd.graph import MultilayerGraph, batch_spec, large_request_spec
-from pargridcpu_amd.hipgrid import REQ_DECOMPOSE, REQ_DENSEST, Engine, HgError, Server
+from pargridcpu_amd.hipgrid import REQ_DECOMPOSE, REQ_DENSEST, Engine, HgError, Server, lib
 from tests import graphs as G
 
 pytestmark = pytest.mark.gpu
@@ -180,3 +180,126 @@ def test_large_requests_in_the_mix():
         assert sizes[True] == [1, 1]
         st = srv.stats()
         assert st["completed"] == len(plan) and st["failed"] == 0
+
+
+@pytest.mark.parametrize("reuse", [True, False])
+def test_resident_graphs(reuse):
+    """Registered graphs stay resident: repeat decompositions and densest
+    searches on them (by handle, nothing uploaded) equal the oracle, mixed
+    with copied-in requests, whether the resident decomposition is reused
+    or recomputed; their bytes count against the budget; a graph with
+    queued requests cannot be unregistered."""
+    gs = request_graphs(6, 0xD7)
+    extra = request_graphs(4, 0xD8)
+    say(f"case server resident graphs reuse={reuse}")
+    with Server(lanes=3, max_batch=4, min_batch=2, reuse_results=reuse) as srv:
+        gids = [srv.register(g) for g in gs]
+        st = srv.stats()
+        assert st["resident_graphs"] == len(gs) and st["resident_bytes"] > 0
+        plan = []
+        for rep in range(3):
+            for gid, g in zip(gids, gs):
+                kind = REQ_DENSEST if (rep + gid) % 3 == 0 else REQ_DECOMPOSE
+                plan.append((srv.submit_resident(gid, kind, 1.0 if rep else 0.5), g, kind, 1.0 if rep else 0.5))
+        for g in extra:
+            plan.append((srv.submit(g, REQ_DECOMPOSE, 1.0), g, REQ_DECOMPOSE, 1.0))
+        for rid, g, kind, beta in plan:
+            check(srv, rid, g, kind, beta, with_alone=False)
+        srv.drain()
+        for gid in gids:
+            srv.unregister(gid)
+        st = srv.stats()
+        assert st["completed"] == len(plan) and st["failed"] == 0
+        assert st["resident_graphs"] == 0 and st["resident_bytes"] == 0
+        with pytest.raises(HgError) as e:
+            srv.unregister(gids[0])
+        assert e.value.code == -1
+
+
+def test_register_respects_budget():
+    gs = request_graphs(2, 0xD9)
+    say("case server register over budget")
+    with Server(lanes=1, hbm_budget=64 << 20) as srv:
+        big = MultilayerGraph.generate(dataclasses.replace(batch_spec(0, base=0xDA), n=1_000_000, L=6,
+                                                           m_lo=4_000_000, m_hi=4_000_000))
+        with pytest.raises(HgError) as e:
+            srv.register(big)
+        assert e.value.code == -3
+        gid = srv.register(gs[0])
+        check(srv, srv.submit_resident(gid), gs[0], REQ_DECOMPOSE, 1.0)
+
+
+def test_cancel_queued_requests():
+    """With one lane busy on a large request, queued requests can be
+    cancelled: they end with HG_ERR_CANCELLED and the rest are answered."""
+    big = MultilayerGraph.generate(dataclasses.replace(batch_spec(0, base=0xDB), n=2_000_000, L=8,
+                                                       m_lo=6_000_000, m_hi=6_000_000))
+    gs = request_graphs(5, 0xDC)
+    say("case server cancel")
+    with Server(lanes=1, plain=True) as srv:
+        gid = srv.register(gs[0])
+        first = srv.submit(big)
+        ids = [srv.submit(g) for g in gs]
+        on_resident = srv.submit_resident(gid)
+        with pytest.raises(HgError) as e:  # queued behind the large request
+            srv.unregister(gid)
+        assert e.value.code == -5
+        srv.cancel(on_resident)
+        srv.unregister(gid)
+        cancelled = []
+        for rid in ids[1::2]:
+            try:
+                srv.cancel(rid)
+                cancelled.append(rid)
+            except HgError as e:  # it had started already (the lane was quicker)
+                assert e.code == -5
+        assert cancelled, "the lane finished a 2M-vertex request before the next submit"
+        for rid, g in zip(ids, gs):
+            if rid in cancelled:
+                with pytest.raises(HgError) as e:
+                    srv.wait(rid)
+                assert e.value.code == -7
+                srv.cancel(rid)  # again: a no-op
+            else:
+                check(srv, rid, g, REQ_DECOMPOSE, 1.0, with_alone=False)
+        assert srv.wait(first).status == 0
+        srv.drain()
+        st = srv.stats()
+        assert st["cancelled"] == len(cancelled) + 1
+        assert st["completed"] == len(gs) + 1 - len(cancelled)
+        with pytest.raises(HgError) as e:
+            srv.cancel(first)
+        assert e.value.code == -5
+
+
+def test_injected_allocation_failure():
+    """A device allocation that fails (injected) fails only the request or
+    batch that made it, with HG_ERR_NOMEM; the loop keeps serving and the
+    other answers equal the oracle."""
+    gs = request_graphs(8, 0xDD)
+    say("case server injected allocation failure")
+    L = lib()
+    try:
+        with Server(lanes=1, plain=True) as srv:
+            assert L.hg_debug_fail_alloc(1) == 0  # the next allocation: the first request's set-up
+            ids = [srv.submit(g) for g in gs[:3]]
+            with pytest.raises(HgError) as e:
+                srv.wait(ids[0])
+            assert e.value.code == -3
+            for rid, g in zip(ids[1:], gs[1:3]):
+                check(srv, rid, g, REQ_DECOMPOSE, 1.0, with_alone=False)
+        with Server(lanes=2, max_batch=8, min_batch=2, large_entries=1 << 40) as srv:
+            assert L.hg_debug_fail_alloc(6) == 0  # somewhere inside the first batch's set-up
+            ids = [srv.submit(g) for g in gs]
+            failed = 0
+            for rid, g in zip(ids, gs):
+                try:
+                    check(srv, rid, g, REQ_DECOMPOSE, 1.0, with_alone=False)
+                except HgError as e:
+                    assert e.code == -3
+                    failed += 1
+            assert failed >= 1
+            later = srv.submit(gs[0])
+            check(srv, later, gs[0], REQ_DECOMPOSE, 1.0, with_alone=False)
+    finally:
+        L.hg_debug_fail_alloc(0)
