@@ -270,13 +270,20 @@ def run_serving(n_requests: int = 160, rates=(20.0, 200.0)) -> dict:
     """Serving form: Poisson arrivals, 80% decompositions and 20%
     densest-subgraph searches; graphs from the batched workload's
     distribution, 10% of them large (n = 5M, L = 12, 60M sampled edges);
-    200 GB device-memory budget.  The request loop (4 lanes: requests of at
-    most 4M directed entries batched when 8 or more wait, all others run
-    alone on the single-graph engine)
-    against one-at-a-time execution on the single-graph engine, same arrival
-    trace, at each offered rate (20 req/s: the GPU is not saturated; 200
-    req/s: arrivals outpace one-at-a-time service and the queue builds up).
-    Every answer must be identical in both."""
+    200 GB device-memory budget.  Same arrival trace in every mode, at each
+    offered rate (20 req/s: the GPU is not saturated; 200 req/s: arrivals
+    outpace one-at-a-time service and the queue builds up):
+      continuous          the request loop (4 lanes: requests of at most 4M
+                          directed entries batched when 8 or more wait, all
+                          others alone on the single-graph engine), every
+                          request copied in and uploaded;
+      one_at_a_time       one request at a time on the single-graph engine;
+      resident            the trace's 50 graphs registered first (resident in
+                          HBM), requests by handle, the resident
+                          decomposition reused by repeat and densest requests;
+      resident_recompute  the same, but every request decomposes again.
+    Every answer (core index and densest result) must be identical in all
+    modes."""
     import math
 
     import numpy as np
@@ -300,29 +307,38 @@ def run_serving(n_requests: int = 160, rates=(20.0, 200.0)) -> dict:
             trace.append((t, g, kind))
         return trace
 
-    def serve(trace, **kw):
+    def serve(trace, resident=False, **kw):
         with Server(hbm_budget=200 << 30, **kw) as srv:
+            gid = {}
+            if resident:  # registered before the clock starts: uploaded once, resident in HBM
+                for g in small + large:
+                    gid[id(g)] = srv.register(g)
             start = time.perf_counter()
             ids = []
             for ta, g, kind in trace:
                 now = time.perf_counter() - start
                 if ta > now:
                     time.sleep(ta - now)
-                ids.append(srv.submit(g, kind, 1.0))
+                ids.append(srv.submit_resident(gid[id(g)], kind, 1.0) if resident else srv.submit(g, kind, 1.0))
             srv.drain()
             st = srv.stats()
             infos = [srv.wait(i) for i in ids]
             outs = [srv.result(i)[0] for i in ids]
-        return st, infos, outs
+            wds = [(float(i.wds.max_density), i.wds.res_k, i.wds.res_lmd, i.wds.res_len) for i in infos]
+        return st, infos, outs, wds
 
+    modes = (("continuous", dict(lanes=4, max_batch=64, large_entries=4_000_000)),
+             ("one_at_a_time", dict(plain=True)),
+             ("resident", dict(lanes=4, resident=True, reuse_results=True)),
+             ("resident_recompute", dict(lanes=4, resident=True, reuse_results=False)))
     by_rate = []
     for rate in rates:
         trace = make_trace(rate)
         res = {"offered_rate": rate}
-        outs0 = None
-        for name, kw in (("continuous", dict(lanes=4, max_batch=64, large_entries=4_000_000)),
-                         ("one_at_a_time", dict(plain=True))):
-            st, infos, outs = serve(trace, **kw)
+        outs0 = wds0 = None
+        same = True
+        for name, kw in modes:
+            st, infos, outs, wds = serve(trace, **kw)
             sizes = [i.batch_size for i in infos]
             res[name] = {"p50_ms": round(st["p50_ms"], 2), "p99_ms": round(st["p99_ms"], 2),
                          "req_per_s": round(n_requests / st["span_s"], 2), "completed": st["completed"],
@@ -330,9 +346,10 @@ def run_serving(n_requests: int = 160, rates=(20.0, 200.0)) -> dict:
                          "mean_batch": round(sum(sizes) / len(sizes), 2)}
             log(f"[bench] serving {rate:g} req/s {name}: {res[name]}")
             if outs0 is None:
-                outs0 = outs
+                outs0, wds0 = outs, wds
             else:
-                res["identical"] = all(np.array_equal(a, c) for a, c in zip(outs0, outs))
+                same = same and all(np.array_equal(a, c) for a, c in zip(outs0, outs)) and wds == wds0
+        res["identical"] = same
         c, p = res["continuous"], res["one_at_a_time"]
         res["p99_ratio"] = round(p["p99_ms"] / c["p99_ms"], 2) if c["p99_ms"] else None
         by_rate.append(res)
