@@ -250,6 +250,35 @@ int hg_batch_info(hg_batch* b, int32_t* count, int64_t* items, int64_t* directed
 
 int hg_batch_destroy(hg_batch* b);
 
+/* ------------------------------------------------------------- live queue
+ * A device work queue shared by batches: items of a batch submitted while
+ * items of other batches are on the GPU are taken by the workgroups already
+ * running (each takes the next item whenever it finishes one), so arrivals
+ * join the running launch as its workgroups free up instead of waiting for
+ * it to end.  A workgroup that finds the queue empty exits; every
+ * submission also makes a launch of its own, so its items always run.
+ * Per item the work and the results are those of hg_batch_decompose
+ * (level 0 is pulled by the item's workgroup).  The reference has no
+ * counterpart (one graph per process, main.cpp:229-257). */
+typedef struct hg_live hg_live;
+
+/* Creates a queue of `slots` item slots (<= 0: 65536) whose launches use
+ * p's block, unroll and blocks_per_cu (NULL: defaults). */
+int hg_live_create(const hg_batch_params* p, int32_t slots, hg_live** out);
+/* Appends every item of b to the queue and returns; b's results are valid
+ * after hg_batch_wait_live.  One submission at a time per batch; the batch
+ * must not be decomposed or destroyed meanwhile (destroy waits). */
+int hg_batch_submit_live(hg_batch* b, hg_live* q);
+/* Blocks until b's submitted items are done; then hg_batch_core_index,
+ * hg_batch_cells and hg_batch_densest work as after hg_batch_decompose,
+ * and hg_batch_last_ms reports submit -> done wall time.  *taken_running
+ * (may be NULL) = its items taken by launches that were already on the GPU
+ * when b was submitted. */
+int hg_batch_wait_live(hg_batch* b, int64_t* taken_running);
+/* Items done, items taken by earlier launches, launches made. */
+int hg_live_stats(hg_live* q, int64_t* items_done, int64_t* items_taken_running, int64_t* launches);
+int hg_live_destroy(hg_live* q);
+
 /* ------------------------------------------------------------------ serving
  * A request loop over one GPU.  Requests (a graph and "decompose" or
  * "densest") are copied in at submit and queue in arrival order; `lanes`
@@ -280,7 +309,10 @@ typedef struct hg_server_config {
     int32_t reuse_results;  /* 1: a registered graph's resident decomposition answers repeat
                                decompositions and densest searches on it; 0: every request
                                decomposes again */
-    int32_t reserved;
+    int32_t device_queue;   /* 1: small requests go, as soon as a lane takes them (up to
+                               max_batch at once, no min_batch wait), to one device work queue
+                               (hg_live) whose running workgroups take their items as they free
+                               up; 0: host-formed batches, each its own launch */
 } hg_server_config;
 
 #define HG_REQ_DECOMPOSE 0

@@ -19,6 +19,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
+#include <mutex>
+#include <thread>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -84,6 +87,11 @@ struct hg_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipEvent_t ev[2] = {nullptr, nullptr};
+    // a submission to a live queue (hg_batch_submit_live) not yet waited for
+    hg_live* live = nullptr;
+    uint64_t live_first = 0, live_count = 0, live_instance = 0;
+    std::chrono::steady_clock::time_point live_t0;
+    int64_t live_taken_running = 0;  // of its items, those taken by launches made before it
     // results
     bool decomposed = false;
     std::vector<int32_t> h_kmax;  // by item id
@@ -781,7 +789,272 @@ int densest_impl(hg_batch* b, const int32_t* graphs, int32_t count, double beta,
 
 }  // namespace
 
+// A device work queue shared by batches (see k_live_peel): a ring of slots
+// in device memory, head/tail words, done flags in host memory that the
+// workgroups raise, and the streams its launches and copies go on.
+struct hg_live {
+    int device = 0;
+    int cus = 0;
+    hg_batch_params params{};
+    uint32_t cap = 0;  // ring slots
+    LiveCtl* ctl = nullptr;
+    LiveSlot* ring = nullptr;
+    unsigned long long* flags_h = nullptr;  // [cap][2]: done position + 1, the launch that took it
+    unsigned long long* flags_d = nullptr;
+    hipStream_t copy = nullptr;
+    std::vector<hipStream_t> ks;  // launch streams, used in turn
+    size_t next_ks = 0;
+    hipEvent_t published = nullptr;
+    std::mutex mu;
+    uint64_t tail = 0;      // positions handed out
+    uint64_t instance = 0;  // launches made
+    std::vector<LiveSlot> staging;
+    int64_t items_done = 0, items_taken_running = 0;
+};
+
+namespace {
+
+void live_release(hg_live* q) {
+    if (!q) return;
+    (void)hipSetDevice(q->device);
+    for (hipStream_t st : q->ks)
+        if (st) (void)hipStreamSynchronize(st);
+    if (q->copy) (void)hipStreamSynchronize(q->copy);
+    if (q->ctl) hgc::dfree(q->ctl);
+    if (q->ring) hgc::dfree(q->ring);
+    if (q->flags_h) (void)hipHostFree(q->flags_h);
+    for (hipStream_t st : q->ks)
+        if (st) (void)hipStreamDestroy(st);
+    if (q->copy) (void)hipStreamDestroy(q->copy);
+    if (q->published) (void)hipEventDestroy(q->published);
+    delete q;
+}
+
+template <int B, int U>
+void launch_live_t(hg_live* q, hipStream_t st, uint32_t grid, unsigned long long inst) {
+    hipLaunchKernelGGL((k_live_peel<B, U>), dim3(grid), dim3(B), 0, st, q->ctl, q->ring, q->cap, q->flags_d, inst);
+}
+
+template <int B, int U>
+int live_resident_t() {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(&k_live_peel<B, U>), B, 0) !=
+        hipSuccess)
+        return 1;
+    return std::max(nb, 1);
+}
+
+int live_resident_per_cu(int block, int unroll) {
+    switch (block * 10 + unroll) {
+        case 2562: return live_resident_t<256, 2>();
+        case 2564: return live_resident_t<256, 4>();
+        case 5122: return live_resident_t<512, 2>();
+        case 5124: return live_resident_t<512, 4>();
+        case 10242: return live_resident_t<1024, 2>();
+        default: return live_resident_t<1024, 4>();
+    }
+}
+
+volatile unsigned long long* live_flag(hg_live* q, uint64_t pos) {
+    return reinterpret_cast<volatile unsigned long long*>(q->flags_h + 2 * (pos % q->cap));
+}
+
+int submit_live_impl(hg_batch* b, hg_live* q) {
+    HIP_TRY(hipSetDevice(b->device));
+    if (b->device != q->device) return fail(HG_ERR_ARG, "batch and live queue are on different devices");
+    if (b->live) return fail(HG_ERR_STATE, "the batch has a submission not yet waited for");
+    b->decomposed = false;
+    if (!b->items_uploaded) {
+        int rc = upload_items(b);
+        if (rc) return rc;
+    }
+    const uint64_t n = b->h_queue.size();
+    b->live_t0 = std::chrono::steady_clock::now();
+    b->live = q;
+    b->live_count = n;
+    b->live_taken_running = 0;
+    if (n == 0) {
+        b->live_first = 0;
+        return HG_OK;
+    }
+    if (n > q->cap) return fail(HG_ERR_RANGE, "batch has more items than the live queue has slots");
+    // the batch's memory is written on its own stream: settled before any workgroup reads it
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    std::lock_guard<std::mutex> lk(q->mu);
+    const uint64_t first = q->tail;
+    // slots are reused once the item that held them cap positions ago is done
+    for (uint64_t p = first; p < first + n; ++p)
+        if (p >= q->cap) {
+            const uint64_t old = p - q->cap;
+            const auto t0 = std::chrono::steady_clock::now();
+            while (*live_flag(q, old) != old + 1) {
+                if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(300))
+                    return fail(HG_ERR_DEVICE, "live queue: a slot was not released within 300 s");
+                std::this_thread::sleep_for(std::chrono::microseconds(50));
+            }
+        }
+    const unsigned long long inst = ++q->instance;
+    q->staging.resize(n);
+    const BPools P{b->off, b->adj, b->adjz, b->z, b->deg0, b->perm, b->degpool, b->cntpool, b->idxpool,
+                   b->listpool, b->qpool, b->bitpool, b->out, b->d_kmax};
+    for (uint64_t i = 0; i < n; ++i) {
+        LiveSlot& sl = q->staging[i];
+        std::memset(&sl, 0, sizeof sl);
+        sl.seq = first + i;
+        sl.instance = inst;
+        sl.P = P;
+        sl.item = b->h_queue[i];
+        sl.gr = b->graphs[sl.item.g];
+        volatile unsigned long long* f = live_flag(q, first + i);
+        f[0] = 0;
+        f[1] = 0;
+    }
+    hgc::Staging io(q->copy);
+    for (uint64_t i = 0; i < n;) {  // the ring may wrap
+        const uint64_t slot = (first + i) % q->cap;
+        const uint64_t m = std::min<uint64_t>(n - i, q->cap - slot);
+        HIP_TRY(io.h2d(q->ring + slot, q->staging.data() + i, sizeof(LiveSlot) * m));
+        i += m;
+    }
+    const unsigned long long new_tail = first + n;
+    HIP_TRY(io.h2d(&q->ctl->tail, &new_tail, sizeof new_tail));
+    HIP_TRY(hipEventRecord(q->published, q->copy));
+    hipStream_t st = q->ks[q->next_ks++ % q->ks.size()];
+    HIP_TRY(hipStreamWaitEvent(st, q->published, 0));
+    const int bpc = std::min(q->params.blocks_per_cu, live_resident_per_cu(q->params.block, q->params.unroll));
+    const uint32_t grid = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(n, uint64_t(q->cus) * uint64_t(std::max(bpc, 1)))));
+    switch (q->params.block * 10 + q->params.unroll) {
+        case 2562: launch_live_t<256, 2>(q, st, grid, inst); break;
+        case 2564: launch_live_t<256, 4>(q, st, grid, inst); break;
+        case 5122: launch_live_t<512, 2>(q, st, grid, inst); break;
+        case 5124: launch_live_t<512, 4>(q, st, grid, inst); break;
+        case 10242: launch_live_t<1024, 2>(q, st, grid, inst); break;
+        default: launch_live_t<1024, 4>(q, st, grid, inst); break;
+    }
+    HIP_TRY(hipGetLastError());
+    q->tail = new_tail;
+    b->live_first = first;
+    b->live_instance = inst;
+    return HG_OK;
+}
+
+int wait_live_impl(hg_batch* b) {
+    hg_live* q = b->live;
+    if (!q) return fail(HG_ERR_STATE, "the batch has no live submission");
+    HIP_TRY(hipSetDevice(b->device));
+    const auto t0 = std::chrono::steady_clock::now();
+    int64_t running = 0;
+    for (uint64_t p = b->live_first; p < b->live_first + b->live_count; ++p) {
+        volatile unsigned long long* f = live_flag(q, p);
+        while (f[0] != p + 1) {
+            if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(300)) {
+                b->live = nullptr;
+                return fail(HG_ERR_DEVICE, "live queue: the batch's items did not finish within 300 s");
+            }
+            std::this_thread::sleep_for(std::chrono::microseconds(20));
+        }
+        running += f[1] < b->live_instance;  // taken by a launch already on the GPU at submission
+    }
+    b->live = nullptr;
+    int32_t err = 0;
+    HIP_TRY(hipMemcpy(&err, &q->ctl->error, sizeof err, hipMemcpyDeviceToHost));
+    if (err) return fail(HG_ERR_DEVICE, "live queue: a slot did not hold the item it was claimed for");
+    {
+        std::lock_guard<std::mutex> lk(q->mu);
+        q->items_done += int64_t(b->live_count);
+        q->items_taken_running += running;
+    }
+    b->live_taken_running = running;
+    b->h_kmax.assign(b->items.size(), 0);
+    if (!b->items.empty()) {
+        hgc::Staging io(b->stream);
+        HIP_TRY(io.d2h(b->h_kmax.data(), b->d_kmax, sizeof(int32_t) * b->items.size()));
+    }
+    b->decompose_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - b->live_t0).count();
+    b->decomposed = true;
+    return HG_OK;
+}
+
+}  // namespace
+
 extern "C" {
+
+int hg_live_create(const hg_batch_params* p, int32_t slots, hg_live** out) {
+    if (!out) return fail(HG_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    hg_batch_params np = p ? *p : default_params();
+    np.use_graph = 0;
+    np.order = 0;
+    if (!params_valid(np)) return fail(HG_ERR_ARG, "invalid launch parameters");
+    if (slots < 1) slots = 65536;
+    hg_live* q = new (std::nothrow) hg_live();
+    if (!q) return fail(HG_ERR_NOMEM, "host allocation failed");
+    q->params = np;
+    q->cap = uint32_t(slots);
+    auto bail = [&](int rc) {
+        live_release(q);
+        return rc;
+    };
+    if (hipGetDevice(&q->device) != hipSuccess ||
+        hipDeviceGetAttribute(&q->cus, hipDeviceAttributeMultiprocessorCount, q->device) != hipSuccess)
+        return bail(fail(HG_ERR_HIP, "no device"));
+    if (dalloc(&q->ctl, 1) != hipSuccess || dalloc(&q->ring, size_t(q->cap)) != hipSuccess)
+        return bail(fail(HG_ERR_NOMEM, "live queue: device allocation failed"));
+    if (hipHostMalloc(reinterpret_cast<void**>(&q->flags_h), sizeof(unsigned long long) * 2 * q->cap,
+                      hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess)
+        return bail(fail(HG_ERR_NOMEM, "live queue: host allocation failed"));
+    std::memset(q->flags_h, 0, sizeof(unsigned long long) * 2 * q->cap);
+    if (hipHostGetDevicePointer(reinterpret_cast<void**>(&q->flags_d), q->flags_h, 0) != hipSuccess)
+        return bail(fail(HG_ERR_HIP, "live queue: no device address for the done flags"));
+    if (hipStreamCreateWithFlags(&q->copy, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&q->published, hipEventDisableTiming) != hipSuccess)
+        return bail(fail(HG_ERR_HIP, "live queue: stream creation failed"));
+    q->ks.assign(4, nullptr);
+    for (auto& st : q->ks)
+        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess)
+            return bail(fail(HG_ERR_HIP, "live queue: stream creation failed"));
+    if (hipMemsetAsync(q->ctl, 0, sizeof(LiveCtl), q->copy) != hipSuccess || hipStreamSynchronize(q->copy) != hipSuccess)
+        return bail(fail(HG_ERR_HIP, "live queue: initialisation failed"));
+    *out = q;
+    return HG_OK;
+}
+
+int hg_batch_submit_live(hg_batch* b, hg_live* q) {
+    if (!b || !q) return fail(HG_ERR_ARG, "NULL argument");
+    try {
+        return submit_live_impl(b, q);
+    } catch (const std::bad_alloc&) {
+        b->live = nullptr;
+        return fail(HG_ERR_NOMEM, "host allocation failed");
+    }
+}
+
+int hg_batch_wait_live(hg_batch* b, int64_t* taken_running) {
+    if (!b) return fail(HG_ERR_ARG, "batch is NULL");
+    int rc;
+    try {
+        rc = wait_live_impl(b);
+    } catch (const std::bad_alloc&) {
+        rc = fail(HG_ERR_NOMEM, "host allocation failed");
+    }
+    if (taken_running) *taken_running = b->live_taken_running;
+    return rc;
+}
+
+int hg_live_stats(hg_live* q, int64_t* items_done, int64_t* items_taken_running, int64_t* launches) {
+    if (!q) return fail(HG_ERR_ARG, "live queue is NULL");
+    std::lock_guard<std::mutex> lk(q->mu);
+    if (items_done) *items_done = q->items_done;
+    if (items_taken_running) *items_taken_running = q->items_taken_running;
+    if (launches) *launches = int64_t(q->instance);
+    return HG_OK;
+}
+
+int hg_live_destroy(hg_live* q) {
+    if (!q) return fail(HG_ERR_ARG, "live queue is NULL");
+    live_release(q);
+    return HG_OK;
+}
 
 int hg_batch_create(const hg_batch_graph* graphs, int32_t count, void* stream, hg_batch** out) {
     if (!out) return fail(HG_ERR_ARG, "out is NULL");
@@ -969,6 +1242,7 @@ int hg_batch_info(hg_batch* b, int32_t* count, int64_t* items, int64_t* directed
 
 int hg_batch_destroy(hg_batch* b) {
     if (!b) return fail(HG_ERR_ARG, "batch is NULL");
+    if (b->live) (void)wait_live_impl(b);  // its items may still be on the GPU
     release(b);
     return HG_OK;
 }

@@ -596,6 +596,82 @@ __global__ void k_batch_reset(uint32_t* head) {
     if (threadIdx.x == 0) head[0] = 0;
 }
 
+// Pool base pointers of a batch (each item's slices are offsets into them).
+struct BPools {
+    const uint32_t* off;
+    const uint32_t* adj;
+    const uint8_t* adjz;
+    const uint8_t* z;
+    const uint32_t* deg0;
+    const uint32_t* perm;
+    uint32_t* degpool;
+    int32_t* cntpool;
+    int32_t* idxpool;
+    uint32_t* listpool;
+    uint32_t* qpool;
+    uint32_t* bitpool;
+    int32_t* out;
+    int32_t* kmax_out;
+};
+
+// One item's whole k-chain (level 0, then cascade / scan per level) and its
+// results (core-index row in the caller's ids, kmax).  All threads call;
+// sh is free on entry and on return.
+template <int B, int U>
+__device__ void run_item(PeelShared<B>* sh, const BItem& item, const BGraph& gr, const BPools& P, bool pulled) {
+    ItemView v;
+    v.off = P.off + gr.ob;
+    v.adj = P.adj + gr.eb;
+    v.adjz = P.adjz + gr.eb;
+    v.z = P.z + gr.vb;
+    v.deg0 = P.deg0 + gr.db;
+    v.perm = P.perm + gr.vb;
+    v.deg = P.degpool + item.degb;
+    v.cnt = P.cntpool + item.sb;
+    v.idx = P.idxpool + item.sb;
+    v.list = P.listpool + item.sb;
+    v.queue = P.qpool + item.sb;
+    v.lds_bits = gr.n <= kLdsBits;
+    v.bits = v.lds_bits ? sh->pool : P.bitpool + item.bitb;
+    v.n = gr.n;
+    v.L = gr.L;
+    v.lmd = item.lmd;
+    v.magic = gr.lmagic;
+    v.H = gr.H;
+    v.pulled = pulled;
+    v.hub16 = gr.hub16 != 0;
+    {
+        // the bitmap words in LDS (rounded to 16 bytes), the table after them
+        v.hub0 = v.lds_bits ? ((gr.n + 31) / 32 + 3) & ~3u : 0u;
+        const uint32_t cap = (v.hub16 ? 2u : 1u) * (kLdsWords + hub_words<B>() - v.hub0) / gr.L;
+        v.T = gr.n < cap ? gr.n : cap;
+    }
+    v.TL = v.T * gr.L;
+    if (threadIdx.x == 0) {
+        sh->listlen = 0;
+        sh->qhead = 0;
+        sh->qtail = 0;
+        sh->kmax = 0;
+    }
+    __syncthreads();
+    peel_level0<B>(v, sh);
+    // every vertex is removed by level max-degree + 1 at the latest
+    // (item.kcap = max layer degree + 2), which also bounds the loop
+    int32_t k = 0;
+    for (;;) {
+        peel_cascade<B, U>(v, sh, k);
+        if (sh->nalive == 0 || uint32_t(k) + 1 >= item.kcap) break;
+        ++k;
+        peel_scan<B>(v, sh, k);
+    }
+    // every vertex got its index at level <= k; k is the largest index
+    sync_global();
+    for (uint32_t u = threadIdx.x; u < v.n; u += B)
+        P.out[item.outb + v.perm[u]] = int32_t(ld_agent(reinterpret_cast<const uint32_t*>(&v.idx[u])));
+    if (threadIdx.x == 0) P.kmax_out[item.id] = k;
+    __syncthreads();
+}
+
 template <int B, int U>
 __global__ void __launch_bounds__(B) k_batch_peel(const BGraph* graphs, const BItem* items, uint32_t nitems,
                                                   uint32_t* head, const uint32_t* off, const uint32_t* adj,
@@ -604,6 +680,7 @@ __global__ void __launch_bounds__(B) k_batch_peel(const BGraph* graphs, const BI
                                                   int32_t* idxpool, uint32_t* listpool, uint32_t* qpool,
                                                   uint32_t* bitpool, int32_t* out, int32_t* kmax_out) {
     __shared__ PeelShared<B> sh;
+    const BPools P{off, adj, adjz, z, deg0, perm, degpool, cntpool, idxpool, listpool, qpool, bitpool, out, kmax_out};
     for (;;) {
         if (threadIdx.x == 0) sh.item = atomicAdd(head, 1u);
         __syncthreads();
@@ -611,57 +688,7 @@ __global__ void __launch_bounds__(B) k_batch_peel(const BGraph* graphs, const BI
         if (it >= nitems) break;
         const BItem item = items[it];
         const BGraph gr = graphs[item.g];
-        ItemView v;
-        v.off = off + gr.ob;
-        v.adj = adj + gr.eb;
-        v.adjz = adjz + gr.eb;
-        v.z = z + gr.vb;
-        v.deg0 = deg0 + gr.db;
-        v.perm = perm + gr.vb;
-        v.deg = degpool + item.degb;
-        v.cnt = cntpool + item.sb;
-        v.idx = idxpool + item.sb;
-        v.list = listpool + item.sb;
-        v.queue = qpool + item.sb;
-        v.lds_bits = gr.n <= kLdsBits;
-        v.bits = v.lds_bits ? sh.pool : bitpool + item.bitb;
-        v.n = gr.n;
-        v.L = gr.L;
-        v.lmd = item.lmd;
-        v.magic = gr.lmagic;
-        v.H = gr.H;
-        v.pulled = gr.pulled != 0;
-        v.hub16 = gr.hub16 != 0;
-        {
-            // the bitmap words in LDS (rounded to 16 bytes), the table after them
-            v.hub0 = v.lds_bits ? ((gr.n + 31) / 32 + 3) & ~3u : 0u;
-            const uint32_t cap = (v.hub16 ? 2u : 1u) * (kLdsWords + hub_words<B>() - v.hub0) / gr.L;
-            v.T = gr.n < cap ? gr.n : cap;
-        }
-        v.TL = v.T * gr.L;
-        if (threadIdx.x == 0) {
-            sh.listlen = 0;
-            sh.qhead = 0;
-            sh.qtail = 0;
-            sh.kmax = 0;
-        }
-        __syncthreads();
-        peel_level0<B>(v, &sh);
-        // every vertex is removed by level max-degree + 1 at the latest
-        // (item.kcap = max layer degree + 2), which also bounds the loop
-        int32_t k = 0;
-        for (;;) {
-            peel_cascade<B, U>(v, &sh, k);
-            if (sh.nalive == 0 || uint32_t(k) + 1 >= item.kcap) break;
-            ++k;
-            peel_scan<B>(v, &sh, k);
-        }
-        // every vertex got its index at level <= k; k is the largest index
-        sync_global();
-        for (uint32_t u = threadIdx.x; u < v.n; u += B)
-            out[item.outb + v.perm[u]] = int32_t(ld_agent(reinterpret_cast<const uint32_t*>(&v.idx[u])));
-        if (threadIdx.x == 0) kmax_out[item.id] = k;
-        __syncthreads();
+        run_item<B, U>(&sh, item, gr, P, gr.pulled != 0);
     }
 }
 
@@ -717,6 +744,95 @@ __global__ void __launch_bounds__(B) k_batch_edge_hist(const BGraph* graphs, con
             }
             __syncthreads();
         }
+    }
+}
+
+}  // namespace hgb
+
+namespace hgb {
+
+// ------------------------------------------------------------- live queue
+// A device work queue shared by batches (hg_live): the host appends items
+// of any batch to a ring of slots and advances `tail`; every workgroup of
+// every running k_live_peel launch claims the next slot with a CAS on
+// `head` whenever it has finished an item, so items submitted while earlier
+// ones run are taken by the workgroups of the launches already on the GPU
+// as they free up.  A workgroup that finds the queue empty exits at once
+// (nothing waits on the GPU); each submission also launches a grid of its
+// own, so its items run even when every earlier workgroup has exited.
+// head and tail are read and updated only with atomics, which the GPU
+// performs past its L2 caches, where the host's copies of tail land.  An
+// item's memory may have been another item's before (the allocation cache
+// hands freed blocks out again), so a workgroup drops its XCD's cached lines
+// before it reads a slot (acquire) and writes its results back (release)
+// before it raises the slot's done flag in host memory.
+struct alignas(128) LiveCtl {
+    unsigned long long head;
+    uint32_t pad0[30];
+    unsigned long long tail;
+    uint32_t pad1[30];
+    int32_t error;
+    uint32_t pad2[31];
+};
+
+struct alignas(256) LiveSlot {
+    unsigned long long seq;       // queue position this slot was written for
+    unsigned long long instance;  // the launch made for the submission
+    BPools P;
+    BGraph gr;
+    BItem item;
+};
+
+enum LiveErr : int32_t { kLiveErrSlot = 1 };
+
+template <int B, int U>
+__global__ void __launch_bounds__(B) k_live_peel(LiveCtl* ctl, const LiveSlot* ring, uint32_t ring_cap,
+                                                 unsigned long long* flags, unsigned long long instance) {
+    __shared__ PeelShared<B> sh;
+    __shared__ LiveSlot s_slot;
+    __shared__ unsigned long long s_pos;
+    for (;;) {
+        if (threadIdx.x == 0) {
+            unsigned long long pos = ~0ull;
+            for (;;) {  // ends: every failed CAS means another workgroup took a slot
+                const unsigned long long h = atomicAdd(&ctl->head, 0ull);
+                const unsigned long long t = atomicAdd(&ctl->tail, 0ull);
+                if (h >= t) break;
+                if (atomicCAS(&ctl->head, h, h + 1) == h) {
+                    pos = h;
+                    break;
+                }
+            }
+            s_pos = pos;
+        }
+        __syncthreads();
+        const unsigned long long pos = s_pos;
+        if (pos == ~0ull) break;
+        // The host copies a submission's slots before its new tail, on one
+        // stream; a slot whose copy has not landed yet is read again (a
+        // bounded number of times) before it counts as an error.
+        bool ok = false;
+        for (uint32_t attempt = 0; attempt < 4096 && !ok; ++attempt) {
+            if (threadIdx.x < kWave) __threadfence();  // acquire: the CU's and XCD's cached lines are dropped
+            __syncthreads();
+            const uint32_t* src = reinterpret_cast<const uint32_t*>(ring + pos % ring_cap);
+            uint32_t* dst = reinterpret_cast<uint32_t*>(&s_slot);
+            for (uint32_t w = threadIdx.x; w < sizeof(LiveSlot) / 4; w += B) dst[w] = src[w];
+            __syncthreads();
+            ok = s_slot.seq == pos;
+            if (!ok) __builtin_amdgcn_s_sleep(64);
+        }
+        if (ok) run_item<B, U>(&sh, s_slot.item, s_slot.gr, s_slot.P, false);
+        else if (threadIdx.x == 0) atomicCAS(&ctl->error, 0, int32_t(kLiveErrSlot));
+        sync_stores();
+        if (threadIdx.x < kWave) __threadfence();  // release: results and item state reach memory
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned long long* f = flags + 2 * (pos % ring_cap);
+            __hip_atomic_store(f + 1, instance, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            __hip_atomic_store(f, pos + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+        __syncthreads();
     }
 }
 

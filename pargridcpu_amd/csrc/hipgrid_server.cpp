@@ -142,6 +142,7 @@ struct hg_server {
     Clock::time_point first_submit, last_done;
     bool stop = false;
     std::vector<std::thread> lanes;
+    hg_live* live = nullptr;  // device_queue: the shared device work queue
 };
 
 namespace {
@@ -245,7 +246,14 @@ int run_batch(hg_server* s, std::vector<Request*>& rs, hipStream_t stream) {
     else rc = hg_batch_get_params(b, &p);
     p.use_graph = 0;
     if (!rc) rc = hg_batch_set_params(b, &p);
-    if (!rc) rc = hg_batch_decompose(b);
+    if (!rc) {
+        if (s->live) {  // joins the workgroups already running
+            rc = hg_batch_submit_live(b, s->live);
+            if (!rc) rc = hg_batch_wait_live(b, nullptr);
+        } else {
+            rc = hg_batch_decompose(b);
+        }
+    }
     // densest requests of one beta are scored in one launch each
     std::vector<int32_t> ids;
     std::vector<double> betas;
@@ -306,7 +314,7 @@ void lane_main(hg_server* s) {
             }
             Request* first = s->pending.front();
             large = is_large(s, first);
-            if (!large) {  // batch only a queue of at least min_batch small requests
+            if (!large && !s->live) {  // batch only a queue of at least min_batch small requests
                 const int32_t want = std::min(s->cfg.min_batch, std::max(1, s->cfg.max_batch));
                 int32_t small = 0;
                 for (auto it = s->pending.begin(); it != s->pending.end() && small < want && !is_large(s, *it); ++it)
@@ -374,6 +382,15 @@ int hg_server_create(const hg_server_config* cfg, hg_server** out) {
         s->cfg.max_batch = 1;
     }
     HIP_TRY(hipGetDevice(&s->device));
+    if (s->cfg.device_queue && !s->cfg.plain) {
+        hg_batch_params lp{};
+        lp.block = 1024;
+        lp.unroll = 4;
+        lp.blocks_per_cu = 1;
+        if (s->cfg.params.block) lp = s->cfg.params;
+        int rc = hg_live_create(&lp, 0, &s->live);
+        if (rc) return rc;
+    }
     try {
         for (int i = 0; i < s->cfg.lanes; ++i) s->lanes.emplace_back(lane_main, s.get());
     } catch (const std::exception&) {
@@ -383,6 +400,7 @@ int hg_server_create(const hg_server_config* cfg, hg_server** out) {
         }
         s->cv_work.notify_all();
         for (auto& t : s->lanes) t.join();
+        if (s->live) hg_live_destroy(s->live);
         return fail(HG_ERR_NOMEM, "cannot start lane threads");
     }
     *out = s.release();
@@ -679,6 +697,7 @@ int hg_server_destroy(hg_server* s) {
         hg_graph_destroy(kv.second->g);
         (void)hipStreamDestroy(kv.second->stream);
     }
+    if (s->live) hg_live_destroy(s->live);
     delete s;
     return HG_OK;
 }

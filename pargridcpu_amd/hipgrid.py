@@ -64,7 +64,7 @@ class BatchParams(C.Structure):
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget", C.c_int64), ("large_entries", C.c_int64), ("lanes", C.c_int32), ("max_batch", C.c_int32),
                 ("plain", C.c_int32), ("min_batch", C.c_int32), ("params", BatchParams), ("reuse_results", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("device_queue", C.c_int32)]
 
 
 class RequestInfo(C.Structure):
@@ -120,6 +120,11 @@ SIGNATURES = {
     "hg_batch_info": ([_P, C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)],
                       C.c_int),
     "hg_batch_destroy": ([_P], C.c_int),
+    "hg_live_create": ([C.POINTER(BatchParams), C.c_int32, C.POINTER(_P)], C.c_int),
+    "hg_batch_submit_live": ([_P, _P], C.c_int),
+    "hg_batch_wait_live": ([_P, C.POINTER(C.c_int64)], C.c_int),
+    "hg_live_stats": ([_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int),
+    "hg_live_destroy": ([_P], C.c_int),
     "hg_server_create": ([C.POINTER(ServerConfig), C.POINTER(_P)], C.c_int),
     "hg_server_submit": ([_P, C.POINTER(BatchGraph), C.c_int32, C.c_double, C.POINTER(C.c_int64)], C.c_int),
     "hg_server_register": ([_P, C.POINTER(BatchGraph), C.POINTER(C.c_int64)], C.c_int),
@@ -402,6 +407,17 @@ class Batch:
         _check(lib().hg_batch_decompose(self._h))
         return self.last_ms()[0]
 
+    def submit_live(self, q: "Live") -> None:
+        """Appends every item of the batch to the live queue q (returns at once)."""
+        _check(lib().hg_batch_submit_live(self._h, q._h))
+
+    def wait_live(self) -> int:
+        """Waits for the live submission; returns how many of its items were
+        taken by launches already on the GPU when it was submitted."""
+        t = C.c_int64()
+        _check(lib().hg_batch_wait_live(self._h, C.byref(t)))
+        return t.value
+
     def last_ms(self):
         a, b = C.c_double(), C.c_double()
         _check(lib().hg_batch_last_ms(self._h, C.byref(a), C.byref(b)))
@@ -435,13 +451,47 @@ class Batch:
         return {"graphs": c.value, "items": it.value, "directed_entries": D.value, "device_bytes": b.value}
 
 
+class Live:
+    """Device work queue shared by batches (hg_live handle): a batch
+    submitted while others run joins the workgroups already on the GPU."""
+
+    def __init__(self, block: int = 1024, unroll: int = 4, blocks_per_cu: int = 1, slots: int = 0):
+        p = BatchParams()
+        p.block, p.unroll, p.blocks_per_cu, p.order, p.use_graph = block, unroll, blocks_per_cu, 0, 0
+        h = _P()
+        _check(lib().hg_live_create(C.byref(p), int(slots), C.byref(h)))
+        self._h = h
+
+    def stats(self) -> dict:
+        d, r, n = C.c_int64(), C.c_int64(), C.c_int64()
+        _check(lib().hg_live_stats(self._h, C.byref(d), C.byref(r), C.byref(n)))
+        return {"items_done": d.value, "items_taken_running": r.value, "launches": n.value}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            _check(lib().hg_live_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
 class Server:
     """Request loop over one GPU (hg_server handle): submit graphs for
     decomposition or densest-subgraph search, wait for their answers."""
 
     def __init__(self, hbm_budget: int = 200 << 30, lanes: int = 4, max_batch: int = 64,
                  large_entries: int = 4_000_000, plain: bool = False, params: dict | None = None,
-                 min_batch: int = 0, reuse_results: bool = True):
+                 min_batch: int = 0, reuse_results: bool = True, device_queue: bool = False):
         cfg = ServerConfig()
         cfg.hbm_budget = int(hbm_budget)
         cfg.large_entries = int(large_entries)
@@ -450,6 +500,7 @@ class Server:
         cfg.plain = 1 if plain else 0
         cfg.min_batch = int(min_batch)
         cfg.reuse_results = 1 if reuse_results else 0
+        cfg.device_queue = 1 if device_queue else 0
         for k, v in (params or {}).items():
             setattr(cfg.params, k, int(v))
         h = _P()
@@ -527,4 +578,4 @@ class Server:
         self.close()
 
 
-__all__ = ["Batch", "Engine", "Server", "FCCoreTree", "HipGrid", "HgError", "NativeLibraryMissing", "SIGNATURES"]
+__all__ = ["Batch", "Engine", "Live", "Server", "FCCoreTree", "HipGrid", "HgError", "NativeLibraryMissing", "SIGNATURES"]
