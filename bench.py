@@ -277,11 +277,18 @@ def run_serving(n_requests: int = 160, rates=(20.0, 200.0)) -> dict:
                           directed entries batched when 8 or more wait, all
                           others alone on the single-graph engine), every
                           request copied in and uploaded;
+      device_queue        8 lanes hand requests of at most 4M entries (up to 8
+                          at once) to one device work queue whose running
+                          workgroups take their items as they free up; larger
+                          ones alone on the single-graph engine;
       one_at_a_time       one request at a time on the single-graph engine;
       resident            the trace's 50 graphs registered first (resident in
                           HBM), requests by handle, the resident
                           decomposition reused by repeat and densest requests;
-      resident_recompute  the same, but every request decomposes again.
+      resident_recompute  the same, but every request decomposes again;
+      resident_device_queue  the same (every request decomposes again), with
+                          the small graphs kept as one-graph batches whose
+                          items join the device work queue.
     Every answer (core index and densest result) must be identical in all
     modes."""
     import math
@@ -328,9 +335,12 @@ def run_serving(n_requests: int = 160, rates=(20.0, 200.0)) -> dict:
         return st, infos, outs, wds
 
     modes = (("continuous", dict(lanes=4, max_batch=64, large_entries=4_000_000)),
+             ("device_queue", dict(lanes=8, max_batch=8, large_entries=4_000_000, device_queue=True)),
              ("one_at_a_time", dict(plain=True)),
              ("resident", dict(lanes=4, resident=True, reuse_results=True)),
-             ("resident_recompute", dict(lanes=4, resident=True, reuse_results=False)))
+             ("resident_recompute", dict(lanes=4, resident=True, reuse_results=False)),
+             ("resident_device_queue", dict(lanes=8, max_batch=8, large_entries=4_000_000, resident=True,
+                                            reuse_results=False, device_queue=True)))
     by_rate = []
     for rate in rates:
         trace = make_trace(rate)

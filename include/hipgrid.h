@@ -342,9 +342,11 @@ typedef struct hg_server_summary {
 int hg_server_create(const hg_server_config* cfg, hg_server** out);
 /* Queues a request (the graph is copied); *id names it. */
 int hg_server_submit(hg_server* s, const hg_batch_graph* graph, int32_t kind, double beta, int64_t* id);
-/* Registers a graph: it is uploaded and laid out once (by the single-graph
- * engine, on a stream of its own) and stays resident in device memory, its
- * bytes counted against hbm_budget, until hg_server_unregister.  Fails with
+/* Registers a graph: it is uploaded and laid out once (on a stream of its
+ * own) and stays resident in device memory, its bytes counted against
+ * hbm_budget, until hg_server_unregister.  It is kept for the single-graph
+ * engine, or, with device_queue and at most large_entries entries, as a
+ * one-graph batch whose items join the device work queue.  Fails with
  * HG_ERR_NOMEM when the registered graphs would exceed the budget.  The
  * caller's arrays may be reused on return. */
 int hg_server_register(hg_server* s, const hg_batch_graph* graph, int64_t* gid);

@@ -84,7 +84,8 @@ struct Request {
 // and the host copy of its last decomposition.
 struct Resident {
     int64_t gid = 0;
-    hg_graph* g = nullptr;
+    hg_graph* g = nullptr;   // single-graph engine handle, or
+    hg_batch* rb = nullptr;  // (device_queue, graphs of at most large_entries entries) a one-graph batch
     hipStream_t stream = nullptr;
     int64_t n = 0;
     int32_t L = 0;
@@ -206,14 +207,17 @@ int run_resident(hg_server* s, Request* r) {
     int rc = HG_OK;
     if (!res->decomposed || !s->cfg.reuse_results) {
         res->decomposed = false;
-        rc = hg_decompose(res->g);
-        if (!rc) {
-            if (!res->ci) res->ci.reset(new int32_t[size_t(res->n) * size_t(res->L)]);
-            rc = hg_core_index(res->g, res->ci.get());
-        }
-        if (!rc) {
-            res->kmax.resize(size_t(res->L));
-            rc = hg_cells(res->g, nullptr, 0, res->kmax.data());
+        if (!res->ci) res->ci.reset(new int32_t[size_t(res->n) * size_t(res->L)]);
+        res->kmax.resize(size_t(res->L));
+        if (res->rb) {  // the graph's items join the device work queue
+            rc = hg_batch_submit_live(res->rb, s->live);
+            if (!rc) rc = hg_batch_wait_live(res->rb, nullptr);
+            if (!rc) rc = hg_batch_core_index(res->rb, 0, res->ci.get());
+            if (!rc) rc = hg_batch_cells(res->rb, 0, nullptr, 0, res->kmax.data());
+        } else {
+            rc = hg_decompose(res->g);
+            if (!rc) rc = hg_core_index(res->g, res->ci.get());
+            if (!rc) rc = hg_cells(res->g, nullptr, 0, res->kmax.data());
         }
         if (rc) return rc;
         res->decomposed = true;
@@ -221,7 +225,14 @@ int run_resident(hg_server* s, Request* r) {
     r->core_index.reset(new int32_t[size_t(res->n) * size_t(res->L)]);
     hgc::par_memcpy(r->core_index.get(), res->ci.get(), 4 * size_t(res->n) * size_t(res->L));
     r->kmax = res->kmax;
-    if (r->kind == HG_REQ_DENSEST) rc = hg_densest(res->g, r->beta, &r->wds);
+    if (r->kind == HG_REQ_DENSEST) {
+        if (res->rb) {
+            const int32_t g0 = 0;
+            rc = hg_batch_densest(res->rb, &g0, 1, r->beta, &r->wds);
+        } else {
+            rc = hg_densest(res->g, r->beta, &r->wds);
+        }
+    }
     return rc;
 }
 
@@ -488,12 +499,27 @@ int hg_server_register(hg_server* s, const hg_batch_graph* graph, int64_t* gid) 
     if (!res) return fail(HG_ERR_NOMEM, "host allocation failed");
     HIP_TRY(hipSetDevice(s->device));
     HIP_TRY(hipStreamCreateWithFlags(&res->stream, hipStreamNonBlocking));
-    int rc = hg_graph_create(graph->layers, graph->L, graph->n, graph->offset_bits, res->stream, &res->g);
+    // entries, to pick the form: with a device queue, graphs the loop would
+    // batch are kept as one-graph batches whose items join the queue
+    int64_t entries = 0;
+    for (int l = 0; l < graph->L; ++l)
+        entries += graph->offset_bits == 64 ? static_cast<const int64_t*>(graph->layers[l].row_ptr)[graph->n]
+                                            : int64_t(static_cast<const int32_t*>(graph->layers[l].row_ptr)[graph->n]);
+    int rc;
     int64_t D = 0, bytes = 0;
-    if (!rc) rc = hg_graph_info(res->g, nullptr, nullptr, &D, &bytes);
+    if (s->live && entries <= s->cfg.large_entries) {
+        rc = hg_batch_create(graph, 1, res->stream, &res->rb);
+        int32_t cnt = 0;
+        int64_t items = 0;
+        if (!rc) rc = hg_batch_info(res->rb, &cnt, &items, &D, &bytes);
+    } else {
+        rc = hg_graph_create(graph->layers, graph->L, graph->n, graph->offset_bits, res->stream, &res->g);
+        if (!rc) rc = hg_graph_info(res->g, nullptr, nullptr, &D, &bytes);
+    }
     if (rc) {
         const std::string err = hg_last_error();
         if (res->g) hg_graph_destroy(res->g);
+        if (res->rb) hg_batch_destroy(res->rb);
         (void)hipStreamDestroy(res->stream);
         return fail(rc, err);
     }
@@ -515,7 +541,8 @@ int hg_server_register(hg_server* s, const hg_batch_graph* graph, int64_t* gid) 
     res->bytes = bytes + int64_t(8 * uint64_t(graph->L) * (maxdeg + 2));
     std::lock_guard<std::mutex> lk(s->mu);
     if (s->resident_bytes + res->bytes > s->cfg.hbm_budget) {
-        hg_graph_destroy(res->g);
+        if (res->g) hg_graph_destroy(res->g);
+        if (res->rb) hg_batch_destroy(res->rb);
         (void)hipStreamDestroy(res->stream);
         return fail(HG_ERR_NOMEM, "registered graphs would exceed the device-memory budget");
     }
@@ -572,7 +599,8 @@ int hg_server_unregister(hg_server* s, int64_t gid) {
         s->resident_bytes -= res->bytes;
         s->cv_work.notify_all();
     }
-    hg_graph_destroy(res->g);
+    if (res->g) hg_graph_destroy(res->g);
+    if (res->rb) hg_batch_destroy(res->rb);
     (void)hipStreamDestroy(res->stream);
     return HG_OK;
 }
@@ -694,7 +722,8 @@ int hg_server_destroy(hg_server* s) {
     s->cv_work.notify_all();
     for (auto& t : s->lanes) t.join();
     for (auto& kv : s->residents) {
-        hg_graph_destroy(kv.second->g);
+        if (kv.second->g) hg_graph_destroy(kv.second->g);
+        if (kv.second->rb) hg_batch_destroy(kv.second->rb);
         (void)hipStreamDestroy(kv.second->stream);
     }
     if (s->live) hg_live_destroy(s->live);

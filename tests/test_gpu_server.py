@@ -303,3 +303,24 @@ def test_injected_allocation_failure():
             check(srv, later, gs[0], REQ_DECOMPOSE, 1.0, with_alone=False)
     finally:
         L.hg_debug_fail_alloc(0)
+
+
+def test_resident_graphs_on_the_device_queue():
+    """With device_queue, small registered graphs are kept as one-graph
+    batches whose items join the shared device queue; large ones on the
+    single-graph engine; every answer equals the oracle's."""
+    gs = request_graphs(8, 0xDE)
+    say("case server resident graphs on the device queue")
+    big = sorted(g.directed_entries() for g in gs)[-2]
+    with Server(lanes=4, max_batch=4, device_queue=True, large_entries=big, reuse_results=False) as srv:
+        gids = [srv.register(g) for g in gs]
+        plan = []
+        for rep in range(2):
+            for gid, g in zip(gids, gs):
+                kind = REQ_DENSEST if (rep + gid) % 3 == 0 else REQ_DECOMPOSE
+                plan.append((srv.submit_resident(gid, kind, 1.0), g, kind, 1.0))
+        for rid, g, kind, beta in plan:
+            check(srv, rid, g, kind, beta, with_alone=False)
+        srv.drain()
+        for gid in gids:
+            srv.unregister(gid)
