@@ -1,7 +1,10 @@
 // hipgrid_main.cpp -- command-line driver with the reference's flags
 // (main.cpp:50-105): -d dataset, -m method, -o layer order, -k/-lmd query,
 // -num_thread, -beta; plus -root <dir> (default "../dataset", the
-// reference's relative dataset root, main.cpp:98).
+// reference's relative dataset root, main.cpp:98) and -phase 1 (times the
+// scans and cascades of the decomposition separately and prints them with
+// the device's busy share, the analogue of the reference's U_overall line,
+// FCSyncLeft.cpp:240-245).
 //
 // Methods (all run on the GPU through include/hipgrid.h):
 //   HipGrid     fcgrid decomposition; replaces -m ParGrid / ParGridPlus /
@@ -43,6 +46,7 @@ int main(int argc, char** argv) {
     std::string dataset = "example", method = "HipGrid", root = "../dataset";
     int order = 0;
     unsigned k = 0, lmd = 0, num_thread = 1;
+    int phase = 0;
     double beta = 1.0;
     for (int i = 1; i < argc; ++i) {
         const std::string a = argv[i];
@@ -55,6 +59,7 @@ int main(int argc, char** argv) {
         else if (a == "-num_thread" && has) num_thread = unsigned(std::atoi(argv[++i]));
         else if (a == "-beta" && has) beta = std::atof(argv[++i]);
         else if (a == "-root" && has) root = argv[++i];
+        else if (a == "-phase" && has) phase = std::atoi(argv[++i]);
     }
     if (method != "HipGrid" && method != "HipGridMem" && method != "HipGridWds" && method != "HipGridWdsCoreIndex") {
         std::fprintf(stderr, "unknown method %s (HipGrid, HipGridMem, HipGridWds, HipGridWdsCoreIndex)\n",
@@ -87,6 +92,7 @@ int main(int argc, char** argv) {
     hg_graph* eng = nullptr;
     if (hg_graph_create(layers.data(), L, n, 64, nullptr, &eng) != HG_OK) return die("hg_graph_create", hg_last_error());
     const double t1 = now_s();
+    if (phase) hg_set_phase_timing(eng, 1);
     if (hg_decompose(eng) != HG_OK) return die("hg_decompose", hg_last_error());
     const double t2 = now_s();
     hg_timing tm{};
@@ -102,6 +108,13 @@ int main(int argc, char** argv) {
     std::printf("device decomposition time: %g ms, levels = %lld, GTEPS = %g\n", tm.decompose_ms,
                 (long long)tm.levels, double(L) * double(D) / (tm.decompose_ms * 1e-3) / 1e9);
     std::printf("count = %lld\n", (long long)cells);
+    if (phase) {
+        // kernels of the level loop over the device time of the decomposition
+        // (the reference's U_overall is the peel threads' busy share)
+        const double busy = tm.init_ms + tm.scan_ms + tm.cascade_ms;
+        std::printf("level 0: %g ms, scans: %g ms, cascades: %g ms\n", tm.init_ms, tm.scan_ms, tm.cascade_ms);
+        std::printf("U_overall = %g %% (device busy in level kernels)\n", 100.0 * busy / tm.decompose_ms);
+    }
 
     if (method == "HipGridMem") {
         int32_t kc = 1;

@@ -146,17 +146,21 @@ def Engine_cells(mg):
 
 
 def test_shells():
+    """Every (k, lmd) shell, FCSyncMem's diff_res (CoreParallel/FCSyncMem.cpp:
+    42-51), against the oracle's core index: the vertices of index k,
+    ascending."""
     mg = MultilayerGraph.generate(small_spec(2500, 4, 7000, 12))
     say("case shells")
     with Engine.from_graph(mg) as eng:
         eng.decompose()
-        ci = eng.core_index()
         off, vtx, kcap = eng.shells()
+    ref = O.coreindex(mg.n, mg.layers())
     stride = kcap + 1
+    assert off[-1] == mg.n * mg.L
     for i in range(mg.L):
         for k in range(kcap):
             a, b = off[i * stride + k], off[i * stride + k + 1]
-            expect = np.nonzero(ci[i] == k)[0]
+            expect = np.nonzero(ref[i] == k)[0]
             assert np.array_equal(vtx[a:b], expect.astype(np.uint32))
 
 

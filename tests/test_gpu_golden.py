@@ -168,10 +168,11 @@ def test_full_size_c2_properties():
     say("case C2 oracle (CPU, ~10 s)")
     ref = O.coreindex(g.n, g.layers())
     assert np.array_equal(a, ref)
-    i = 0
-    for k in (0, 1, int(kmax[0])):
-        s = vtx[off[i * stride + k]:off[i * stride + k + 1]]
-        assert np.array_equal(s, np.nonzero(ref[i] == k)[0].astype(np.uint32))
+    for i in range(g.L):  # every shell: the vertices of each index, ids ascending, k ascending
+        h = np.bincount(ref[i], minlength=kcap)
+        assert np.array_equal(np.diff(off[i * stride:i * stride + kcap + 1]), h[:kcap])
+        seg = vtx[off[i * stride]:off[i * stride + kcap]]
+        assert np.array_equal(seg, np.argsort(ref[i], kind="stable").astype(np.uint32))
 
 
 def _driver_root(tmp_path, dataset):
@@ -234,3 +235,20 @@ def test_driver_mem_shells_match_reference(tmp_path, name):
     want = {(k, i + 1): int(np.count_nonzero(ci[i] == k)) for i in range(ci.shape[0])
             for k in range(1, int(ci[i].max()) + 1)}
     assert got == want
+
+
+def test_driver_phase_timing(tmp_path):
+    """`-phase 1`: the decomposition's level-0, scan and cascade times and
+    the busy share, beside the reference's lines; the counts are unchanged."""
+    fx = fixture("g2000_L5")
+    root = _driver_root(tmp_path, "g2000_L5")
+    exe = os.path.join(ROOT, "pargridcpu_amd", "bin", "hipgrid")
+    say("case driver phase timing")
+    r = subprocess.run([exe, "-d", "g2000_L5", "-m", "HipGrid", "-phase", "1"], cwd=str(root / "run"),
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert re.search(r"count = (\d+)", r.stdout).group(1) == str(int(fx["lpe_count"]))
+    m = re.search(r"level 0: (\S+) ms, scans: (\S+) ms, cascades: (\S+) ms", r.stdout)
+    assert m and all(float(x) >= 0 for x in m.groups())
+    u = float(re.search(r"U_overall = (\S+) %", r.stdout).group(1))
+    assert 0 < u <= 100.5
