@@ -67,12 +67,12 @@ def host_info() -> dict:
     return {"nproc": os.cpu_count(), "cpu_model": model}
 
 
-def run_reference_pargrid(g, threads: int) -> dict:
+def run_reference_pargrid(g, threads: int, method: str = "ParGrid", extra=()) -> dict:
     """Runs the reference's own driver (oracle/_ref/firmcore_ref, built from
-    its sources at -O3) as `-m ParGrid -num_thread threads` on g written in
+    its sources at -O3) as `-m <method> -num_thread threads` on g written in
     the reference's text format; returns its printed elapsed time (its own
-    omp_get_wtime bracket around FCSyncLeft::Execute, main.cpp:229-240) and
-    peak RSS."""
+    omp_get_wtime bracket, main.cpp:183-245 and 402-424) and its peak RSS
+    (VmHWM sampled while it runs, MB)."""
     ref_bin = os.path.join(ROOT, "oracle", "_ref", "firmcore_ref")
     tmp = tempfile.mkdtemp(prefix="hgcpu_")
     try:
@@ -84,8 +84,10 @@ def run_reference_pargrid(g, threads: int) -> dict:
         # the high-water mark of the process it was forked from (this one,
         # holding the benchmark graphs) across exec; its own is VmHWM of its
         # address space, sampled while it runs.
-        proc = subprocess.Popen([ref_bin, "-d", "bench", "-m", "ParGrid", "-num_thread", str(threads)],
-                                cwd=os.path.join(tmp, "run"), stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
+        out_path = os.path.join(tmp, "ref.out")
+        fout = open(out_path, "w")  # a file, not a pipe: lpe prints a line per cell
+        proc = subprocess.Popen([ref_bin, "-d", "bench", "-m", method, "-num_thread", str(threads), *extra],
+                                cwd=os.path.join(tmp, "run"), stdout=fout, stderr=subprocess.STDOUT)
         hwm_kb = 0
         t_end = time.time() + 900
         while proc.poll() is None and time.time() < t_end:
@@ -98,14 +100,18 @@ def run_reference_pargrid(g, threads: int) -> dict:
             time.sleep(0.02)
         if proc.poll() is None:
             proc.kill()
-        stdout, stderr = proc.communicate()
+        proc.wait()
+        fout.close()
+        stdout, stderr = open(out_path).read(), ""
         secs = None
         rss = hwm_kb / 1024.0 if hwm_kb else None
+        marker = {"ParGrid": "Path left Core Parallel Sync Elapsed time", "lpe": "Serial Elapsed time",
+                  "kpe": "Serial Elapsed time", "WdsList": "Wds Fctree Elapsed time"}[method]
         for line in stdout.splitlines():
-            if "Path left Core Parallel Sync Elapsed time" in line:
+            if marker in line:
                 secs = float(line.split(":")[1].split()[0])
         if secs is None:
-            raise RuntimeError("reference ParGrid printed no elapsed time:\n" + stdout[-2000:] + stderr)
+            raise RuntimeError(f"reference {method} printed no elapsed time:\n" + stdout[-2000:] + stderr)
         return {"seconds": secs, "peak_rss_mb": rss}
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
@@ -439,6 +445,45 @@ def run_decompose(args, spec) -> dict:
                        "rho": spec.rho, "offsets": "int64", "parallelism": "single-gpu"}}
 
 
+def host_table() -> dict:
+    """Rows of the CPU-baseline table on this host for the 20k- and
+    1M-vertex workloads at full size: the reference's ParGrid (nproc
+    threads, peel team 32), its serial lpe and kpe, WdsList at beta = 1,
+    each with its own elapsed-time line and peak RSS; the GPU engine on the
+    same graphs (decomposition and densest subgraph)."""
+    from pargridcpu_amd.graph import C1, C2, MultilayerGraph
+    from pargridcpu_amd.hipgrid import Engine
+
+    host = host_info()
+    threads = int(host["nproc"] or 1)
+    rows = []
+    for spec in (C1, C2):
+        g = MultilayerGraph.generate(spec)
+        D = g.directed_entries()
+        for method, extra in (("ParGrid", ()), ("lpe", ()), ("kpe", ()), ("WdsList", ("-beta", "1.0"))):
+            r = run_reference_pargrid(g, threads, method, extra)
+            rows.append({"workload": spec.name, "method": f"reference {method}", "threads": threads,
+                         "seconds": r["seconds"], "peak_rss_mb": r["peak_rss_mb"],
+                         "gteps": round(g.L * D / r["seconds"] / 1e9, 4) if method != "WdsList" else None})
+            log(f"[bench] host table: {rows[-1]}")
+        with Engine.from_graph(g) as eng:
+            eng.decompose()
+            ms, dms = [], []
+            for _ in range(5):
+                eng.decompose()
+                ms.append(eng.timing().decompose_ms)
+                eng.densest(1.0)
+                dms.append(eng.timing().densest_ms)
+            dev = eng.info()["device_bytes"]
+        t = sorted(ms)[2]
+        rows.append({"workload": spec.name, "method": "HipGrid hg_decompose", "seconds": round(t / 1e3, 6),
+                     "gteps": round(g.L * D / (t * 1e-3) / 1e9, 3), "device_bytes": dev})
+        rows.append({"workload": spec.name, "method": "HipGrid hg_densest (beta 1.0)",
+                     "seconds": round(sorted(dms)[2] / 1e3, 6)})
+        log(f"[bench] host table: {rows[-2]} {rows[-1]}")
+    return {"host": host, "rows": rows}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -451,6 +496,9 @@ def main() -> int:
     ap.add_argument("--batch-params", default="tune",
                     help='batched launch shape: "tune" or "block,unroll,blocks_per_cu,order"')
     ap.add_argument("--serving", type=int, default=1, help="also run the serving form (0: skip)")
+    ap.add_argument("--host-table", action="store_true",
+                    help="only print the CPU-baseline table of the 20k- and 1M-vertex workloads (reference modes and "
+                         "the engine, full size) as JSON")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -458,6 +506,10 @@ def main() -> int:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the lambda-sharded multi-GPU "
               f"layout is a design, not built", file=sys.stderr)
         return 2
+
+    if args.host_table:
+        print(json.dumps(host_table()), flush=True)
+        return 0
 
     from pargridcpu_amd.graph import WORKLOADS
 
