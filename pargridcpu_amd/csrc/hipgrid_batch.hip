@@ -72,6 +72,7 @@ struct hg_batch {
     uint32_t ntiles = 0, nheavy = 0, nmed = 0, pull_nj = 0;
     uint32_t* d_head = nullptr;
     int32_t* d_kmax = nullptr;  // by item id
+    unsigned long long* d_clock = nullptr;  // HG_ITEM_CLOCK=1 (diagnostic): item start/end wall clock
     uint32_t* degpool = nullptr;
     int32_t* cntpool = nullptr;
     int32_t* idxpool = nullptr;
@@ -131,7 +132,7 @@ void release(hg_batch* b) {
     drop_graph(b);
     void* ptrs[] = {b->off,     b->adj,     b->adjz,     b->z,        b->deg0,    b->perm,
                     b->out,     b->d_graphs, b->d_items, b->d_items_id, b->d_tiles, b->d_head,
-                    b->d_kmax,  b->degpool, b->cntpool,  b->idxpool,  b->listpool, b->qpool,
+                    b->d_kmax,  b->d_clock, b->degpool, b->cntpool,  b->idxpool,  b->listpool, b->qpool,
                     b->bitpool, b->histpool, b->d_sel};
     for (void* p : ptrs)
         if (p) hgc::dfree(p);
@@ -379,6 +380,7 @@ int create_impl(const hg_batch_graph* gs, int32_t count, void* stream, hg_batch*
     HIP_TRY(dalloc(&b->d_tiles, std::max<size_t>(tiles.size(), 1)));
     HIP_TRY(dalloc(&b->d_head, 1));
     HIP_TRY(dalloc(&b->d_kmax, I));
+    if (env_int("HG_ITEM_CLOCK", 0) == 1) HIP_TRY(dalloc(&b->d_clock, 2 * std::max<size_t>(I, 1)));
     HIP_TRY(dalloc(&b->degpool, SDEG)); acct(4 * SDEG);
     HIP_TRY(dalloc(&b->cntpool, SN)); acct(4 * SN);
     HIP_TRY(dalloc(&b->idxpool, SN)); acct(4 * SN);
@@ -538,7 +540,8 @@ template <int B, int U>
 void launch_peel_t(hg_batch* b, uint32_t grid) {
     hipLaunchKernelGGL((k_batch_peel<B, U>), dim3(grid), dim3(B), 0, b->stream, b->d_graphs, b->d_items,
                        uint32_t(b->items.size()), b->d_head, b->off, b->adj, b->adjz, b->z, b->deg0, b->perm,
-                       b->degpool, b->cntpool, b->idxpool, b->listpool, b->qpool, b->bitpool, b->out, b->d_kmax);
+                       b->degpool, b->cntpool, b->idxpool, b->listpool, b->qpool, b->bitpool, b->out, b->d_kmax,
+                       b->d_clock);
 }
 
 template <int B, int U>
@@ -648,6 +651,31 @@ int decompose_impl(hg_batch* b) {
     b->h_kmax.assign(b->items.size(), 0);
     hgc::Staging io(b->stream);
     HIP_TRY(io.d2h(b->h_kmax.data(), b->d_kmax, sizeof(int32_t) * b->items.size()));
+    if (b->d_clock) {  // diagnostic: item durations and finish times (wall clock of the device)
+        std::vector<unsigned long long> c(2 * b->items.size());
+        HIP_TRY(io.d2h(c.data(), b->d_clock, sizeof(unsigned long long) * c.size()));
+        int khz = 100000;
+        (void)hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, b->device);
+        unsigned long long t0 = ~0ull, t1 = 0;
+        for (size_t i = 0; i < b->items.size(); ++i) {
+            t0 = std::min(t0, c[2 * i]);
+            t1 = std::max(t1, c[2 * i + 1]);
+        }
+        std::vector<std::pair<double, size_t>> dur;
+        for (size_t i = 0; i < b->items.size(); ++i) dur.emplace_back(double(c[2 * i + 1] - c[2 * i]) / khz, i);
+        std::sort(dur.rbegin(), dur.rend());
+        double sum = 0;
+        for (auto& d : dur) sum += d.first;
+        std::fprintf(stderr, "[hg batch] items %zu span %.3f ms sum of item times %.3f ms (%.1f per workgroup of %d)\n",
+                     dur.size(), double(t1 - t0) / khz, sum, sum / std::max(1, b->cus * std::max(1, b->params.blocks_per_cu)),
+                     b->cus * b->params.blocks_per_cu);
+        for (size_t j = 0; j < std::min<size_t>(dur.size(), 20); ++j) {
+            const size_t i = dur[j].second;
+            std::fprintf(stderr, "[hg batch]   item %zu graph %u lmd %u n %u L %u: %.3f ms, starts %.3f ends %.3f\n", i,
+                         b->items[i].g, b->items[i].lmd, b->graphs[b->items[i].g].n, b->graphs[b->items[i].g].L,
+                         dur[j].first, double(c[2 * i] - t0) / khz, double(c[2 * i + 1] - t0) / khz);
+        }
+    }
     b->decomposed = true;
     return HG_OK;
 }
@@ -895,8 +923,8 @@ int submit_live_impl(hg_batch* b, hg_live* q) {
         }
     const unsigned long long inst = ++q->instance;
     q->staging.resize(n);
-    const BPools P{b->off, b->adj, b->adjz, b->z, b->deg0, b->perm, b->degpool, b->cntpool, b->idxpool,
-                   b->listpool, b->qpool, b->bitpool, b->out, b->d_kmax};
+    const BPools P{b->off,     b->adj,      b->adjz,  b->z,       b->deg0, b->perm,  b->degpool, b->cntpool,
+                   b->idxpool, b->listpool, b->qpool, b->bitpool, b->out,  b->d_kmax, nullptr};
     for (uint64_t i = 0; i < n; ++i) {
         LiveSlot& sl = q->staging[i];
         std::memset(&sl, 0, sizeof sl);

@@ -612,6 +612,7 @@ struct BPools {
     uint32_t* bitpool;
     int32_t* out;
     int32_t* kmax_out;
+    unsigned long long* item_clock;  // diagnostic (nullptr: off): [item id][2] start, end (wall clock)
 };
 
 // One item's whole k-chain (level 0, then cascade / scan per level) and its
@@ -619,6 +620,7 @@ struct BPools {
 // sh is free on entry and on return.
 template <int B, int U>
 __device__ void run_item(PeelShared<B>* sh, const BItem& item, const BGraph& gr, const BPools& P, bool pulled) {
+    if (P.item_clock && threadIdx.x == 0) P.item_clock[2 * item.id] = wall_clock64();
     ItemView v;
     v.off = P.off + gr.ob;
     v.adj = P.adj + gr.eb;
@@ -669,6 +671,7 @@ __device__ void run_item(PeelShared<B>* sh, const BItem& item, const BGraph& gr,
     for (uint32_t u = threadIdx.x; u < v.n; u += B)
         P.out[item.outb + v.perm[u]] = int32_t(ld_agent(reinterpret_cast<const uint32_t*>(&v.idx[u])));
     if (threadIdx.x == 0) P.kmax_out[item.id] = k;
+    if (P.item_clock && threadIdx.x == 0) P.item_clock[2 * item.id + 1] = wall_clock64();
     __syncthreads();
 }
 
@@ -678,9 +681,11 @@ __global__ void __launch_bounds__(B) k_batch_peel(const BGraph* graphs, const BI
                                                   const uint8_t* adjz, const uint8_t* z, const uint32_t* deg0,
                                                   const uint32_t* perm, uint32_t* degpool, int32_t* cntpool,
                                                   int32_t* idxpool, uint32_t* listpool, uint32_t* qpool,
-                                                  uint32_t* bitpool, int32_t* out, int32_t* kmax_out) {
+                                                  uint32_t* bitpool, int32_t* out, int32_t* kmax_out,
+                                                  unsigned long long* item_clock) {
     __shared__ PeelShared<B> sh;
-    const BPools P{off, adj, adjz, z, deg0, perm, degpool, cntpool, idxpool, listpool, qpool, bitpool, out, kmax_out};
+    const BPools P{off,     adj,      adjz,  z,   deg0,     perm,      degpool,   cntpool,
+                   idxpool, listpool, qpool, bitpool, out, kmax_out, item_clock};
     for (;;) {
         if (threadIdx.x == 0) sh.item = atomicAdd(head, 1u);
         __syncthreads();
