@@ -36,6 +36,7 @@
 #include "hipgrid.h"
 #include "hipgrid_common.h"
 #include "hipgrid_kernels.h"
+#include "hipgrid_bin_kernels.h"
 
 using namespace hg;
 using hgc::dalloc;
@@ -124,6 +125,17 @@ struct hg_graph {
     int64_t cmp_bytes = 0;
     std::vector<Inst> cur;     // the instances' current rows during hg_decompose
     int64_t compactions = 0;   // in the last hg_decompose
+    // binned apply of dense first rounds (k_bin_*): coarse bin table, per-bin
+    // counts / cursors / region starts per instance, two record buffers
+    bool bin_on = false;
+    uint32_t bin_steps = 0;  // steps 1..bin_steps may bin
+    BinGeo bgeo{};
+    uint32_t* bin_cbw = nullptr;
+    uint32_t* bin_bh = nullptr;
+    uint32_t* bin_bc = nullptr;
+    uint64_t* bin_bb = nullptr;
+    uint32_t* bin_rec = nullptr;  // rec1 then rec2
+    uint16_t* bin_gsel = nullptr;
 };
 
 namespace {
@@ -147,6 +159,9 @@ void release(hg_graph* g) {
             if (c.off[b]) hgc::dfree(c.off[b]);
             if (c.adj[b]) hgc::dfree(c.adj[b]);
         }
+    void* bptrs[] = {g->bin_cbw, g->bin_bh, g->bin_bc, g->bin_bb, g->bin_rec, g->bin_gsel};
+    for (void* p : bptrs)
+        if (p) hgc::dfree(p);
     void* cptrs[] = {g->c_len, g->c_nch, g->c_item_off, g->c_items, g->c_temp, g->c_words};
     for (void* p : cptrs)
         if (p) hgc::dfree(p);
@@ -171,6 +186,86 @@ struct Scratch {
         return e;
     }
 };
+
+// Coarse bins of the binned apply: vertex ranges of about D / C_target
+// entries each (from the merged offsets: hub ranges are short), none longer
+// than kBinMaxF fine bins of vb = kBinDeltaWords / L vertices.  A graph whose
+// table would exceed kBinMaxC bins keeps the atomic path everywhere.
+int bin_setup(hg_graph* g, Scratch& tmp, hgc::Staging& up) {
+    const uint32_t L = g->L, n = g->n;
+    const uint32_t vb = std::min<uint32_t>(kBinDeltaWords / L, kBinMaxVb);
+    const uint64_t vlimit = uint64_t(vb) * kBinMaxF;
+    const uint64_t ct = uint64_t(std::max(1, env_int("HG_BIN_C", 512)));
+    const uint64_t E = std::max<uint64_t>(1, (g->D + ct - 1) / ct);
+    const uint32_t J = uint32_t((g->D + E - 1) / E);
+    uint32_t* bnd = nullptr;
+    HIP_TRY(tmp.alloc(&bnd, size_t(J) + 1));
+    hipLaunchKernelGGL(k_bin_bounds, dim3((J + 1 + 255) / 256), dim3(256), 0, g->stream, g->off, n, E, J, bnd);
+    HIP_TRY(hipGetLastError());
+    std::vector<uint32_t> hb(size_t(J) + 1);
+    HIP_TRY(up.d2h(hb.data(), bnd, sizeof(uint32_t) * hb.size()));
+    std::vector<uint32_t> cb{0};
+    for (uint32_t j = 1; j <= J; ++j) {
+        const uint32_t b = j == J ? n : std::min(hb[j], n);
+        while (uint64_t(b) - cb.back() > vlimit) cb.push_back(uint32_t(cb.back() + vlimit));
+        if (b > cb.back()) cb.push_back(b);
+    }
+    const uint32_t C = uint32_t(cb.size() - 1);
+    if (C < 1 || C > kBinMaxC) return HG_OK;  // binning stays off for this graph
+    std::vector<uint32_t> cbw(cb.size());
+    for (size_t c = 0; c < cb.size(); ++c) cbw[c] = cb[c] * L;
+    // granule table: word granules of 2^gsh words, at most kBinMaxG of them
+    const uint64_t nL = uint64_t(n) * L;
+    uint32_t gsh = 0;
+    while (((nL + (1ull << gsh) - 1) >> gsh) > kBinMaxG) ++gsh;
+    const uint32_t ng = uint32_t((nL + (1ull << gsh) - 1) >> gsh);
+    std::vector<uint16_t> gsel(size_t(ng) + 1);
+    {
+        uint32_t c = 0;
+        for (uint32_t q = 0; q < ng; ++q) {
+            const uint64_t w = uint64_t(q) << gsh;  // first word of granule q
+            while (c + 1 < C && cbw[c + 1] <= w) ++c;
+            gsel[q] = uint16_t(c);
+        }
+        gsel[ng] = uint16_t(C - 1);
+    }
+    const uint64_t rcap = hgc::bin_rcap(g->D);
+    const size_t tab = size_t(L) * C;
+    HIP_TRY(dalloc(&g->bin_cbw, cbw.size()));
+    HIP_TRY(dalloc(&g->bin_bh, tab));
+    HIP_TRY(dalloc(&g->bin_bc, tab));
+    HIP_TRY(dalloc(&g->bin_bb, tab));
+    HIP_TRY(dalloc(&g->bin_rec, size_t(2 * rcap)));
+    HIP_TRY(dalloc(&g->bin_gsel, gsel.size()));
+    HIP_TRY(up.h2d(g->bin_cbw, cbw.data(), sizeof(uint32_t) * cbw.size()));
+    HIP_TRY(up.h2d(g->bin_gsel, gsel.data(), sizeof(uint16_t) * gsel.size()));
+    HIP_TRY(hipMemsetAsync(g->bin_bh, 0, sizeof(uint32_t) * tab, g->stream));
+    HIP_TRY(hipMemsetAsync(g->bin_bc, 0, sizeof(uint32_t) * tab, g->stream));
+    BinGeo& bg = g->bgeo;
+    bg.cbw = g->bin_cbw;
+    bg.bh = g->bin_bh;
+    bg.bc = g->bin_bc;
+    bg.bb = g->bin_bb;
+    bg.rec1 = g->bin_rec;
+    bg.rec2 = g->bin_rec + rcap;
+    bg.rcap = rcap;
+    bg.gsel = g->bin_gsel;
+    bg.ng = ng;
+    bg.gsh = gsh;
+    bg.min_ent = uint64_t(std::max(1, env_int("HG_BIN_MINENT", 1 << 20)));
+    bg.C = C;
+    bg.cmax = C;
+    bg.vbw = vb * L;
+    bg.stage = std::min<uint32_t>(uint32_t(kWave), kBinStageWords / C);
+    // binned when the frontier's entries reach HG_BIN_DENS / 1000 of the
+    // alive vertices' degree words (default 30, i.e. 3 %)
+    const char* dv = std::getenv("HG_BIN_DENS");
+    bg.dens_num = dv && *dv ? uint32_t(std::max(0, std::atoi(dv))) : 30u;
+    bg.dens_den = 1000;
+    g->bin_steps = uint32_t(env_int("HG_BIN_STEPS", 64));
+    g->bin_on = true;
+    return HG_OK;
+}
 
 hipError_t sort_by_total(void* temp, size_t& temp_bytes, const uint64_t* tot, uint64_t* tot_s, const uint32_t* ids,
                          uint32_t* perm, uint32_t n, hipStream_t stream) {
@@ -388,6 +483,11 @@ int create_impl(const hg_layer_csr* layers, int32_t L, int64_t n, int32_t offset
             if (!same) return fail(HG_ERR_HIP, "upload verify: layer " + std::to_string(l) + " differs on the device");
         }
     }
+    if (hgc::bin_enabled(g->D)) {
+        int rc = bin_setup(g, tmp, up);
+        if (rc) return rc;
+        g->device_bytes += int64_t(hgc::bin_bytes(g->D, g->L));
+    }
     // the temporaries are freed on return: nothing queued reads them any more
     HIP_TRY(hipStreamSynchronize(g->stream));
 
@@ -433,6 +533,7 @@ const char* err_name(int32_t e) {
     switch (e) {
         case kErrDefer: return "deferred chunk list overflow";
         case kErrCompact: return "row compaction: copied rows do not match the degrees";
+        case kErrBin: return "binned apply: a record fell outside its bin";
         default: return "unknown device error";
     }
 }
@@ -482,7 +583,60 @@ int launch_step(hg_graph* g, uint32_t s) {
     else
         hipLaunchKernelGGL(k_scan, dim3(g->G), dim3(kBlock), 0, g->stream, g->d_insts, g->d_states, g->d_ctl, L, s);
     if (g->phase_timing) HIP_TRY(hipEventRecord(g->pev[2 * size_t(s) + 1], g->stream));
+    // diagnostic (HG_BIN_LOG=1): every step's kernels timed one by one and
+    // the binned instances' frontier entries, records and removals printed
+    static const bool blog = env_int("HG_BIN_LOG", 0) == 1;
+    double tk[4] = {0, 0, 0, 0};
+    auto tick = [&](int slot, std::chrono::steady_clock::time_point t0) -> int {
+        HIP_TRY(hipStreamSynchronize(g->stream));
+        tk[slot] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        return HG_OK;
+    };
+    std::vector<InstState> before;
+    if (blog) {
+        HIP_TRY(hipStreamSynchronize(g->stream));
+        before.resize(L);
+        HIP_TRY(hipMemcpy(before.data(), g->d_states, sizeof(InstState) * L, hipMemcpyDeviceToHost));
+    }
+    if (s >= 1 && g->bin_on && s <= g->bin_steps) {
+        // dense first rounds: binned apply (the kernels return at once for
+        // instances that do not bin this step)
+        const uint32_t g2 = 2u * uint32_t(g->cus);
+        auto t0 = std::chrono::steady_clock::now();
+        hipLaunchKernelGGL(k_bin_expand, dim3(g->G), dim3(kBlock), 0, g->stream, g->d_insts, g->d_states, g->d_ctl, L,
+                           s, g->bgeo);
+        if (blog && tick(0, t0)) return HG_ERR_HIP;
+        t0 = std::chrono::steady_clock::now();
+        hipLaunchKernelGGL(k_bin_scatter, dim3(g2), dim3(kBlock), 0, g->stream, g->d_insts, g->d_states, g->d_ctl, L,
+                           s, g->bgeo);
+        if (blog && tick(1, t0)) return HG_ERR_HIP;
+        t0 = std::chrono::steady_clock::now();
+        hipLaunchKernelGGL(k_bin_apply, dim3(g2), dim3(kBlock), 0, g->stream, g->d_insts, g->d_states, g->d_ctl, L,
+                           s, g->bgeo);
+        if (blog && tick(2, t0)) return HG_ERR_HIP;
+    }
+    auto t3 = std::chrono::steady_clock::now();
     hipLaunchKernelGGL(k_cascade, dim3(g->G), dim3(kBlock), 0, g->stream, g->d_insts, g->d_states, g->d_ctl, L, s);
+    if (blog) {
+        if (tick(3, t3)) return HG_ERR_HIP;
+        std::vector<InstState> after(L);
+        HIP_TRY(hipMemcpy(after.data(), g->d_states, sizeof(InstState) * L, hipMemcpyDeviceToHost));
+        const uint32_t p = s & 1u;
+        std::string line;
+        for (uint32_t i = 0; i < L; ++i) {
+            const InstState& a = after[i];
+            if (a.dphase[p] == kPhaseDone) continue;
+            char buf[256];
+            std::snprintf(buf, sizeof buf, " | l%u %s k%d f0ent %llu rec %llu f1ch %llu alive %llu", i + 1,
+                          a.dphase[p] == kPhaseScan ? (a.bmode[p] ? "B" : "S") : "C", a.dk[p], a.pfent[p][0],
+                          a.brec[0] - before[i].brec[0], a.bmode[p] ? a.pbin[p][0] : 0ull, a.alive[0]);
+            line += buf;
+        }
+        Control c{};
+        HIP_TRY(hipMemcpy(&c, g->d_ctl, sizeof(Control), hipMemcpyDeviceToHost));
+        std::fprintf(stderr, "[hg] step %u count %.3f scatter %.3f apply %.3f cascade %.3f dbg %llu %llu %llu%s\n", s,
+                     tk[0], tk[1], tk[2], tk[3], c.dbg[0], c.dbg[1], c.dbg[2], line.c_str());
+    }
     return HG_OK;
 }
 
@@ -661,6 +815,11 @@ int decompose_impl(hg_graph* g) {
     }
     HIP_TRY(io.h2d(g->d_states, st.data(), sizeof(InstState) * L));
     HIP_TRY(io.h2d(g->d_ctl, &ctl, sizeof(Control)));
+    if (g->bin_on) {  // the per-bin tables start at zero (an earlier call may have stopped on an error)
+        const size_t tab = size_t(L) * g->bgeo.cmax;
+        HIP_TRY(hipMemsetAsync(g->bin_bh, 0, sizeof(uint32_t) * tab, g->stream));
+        HIP_TRY(hipMemsetAsync(g->bin_bc, 0, sizeof(uint32_t) * tab, g->stream));
+    }
 
     HIP_TRY(hipEventRecord(g->ev[0], g->stream));
     uint32_t s = 0;
@@ -735,7 +894,8 @@ int decompose_impl(hg_graph* g) {
         g->timing.cascade_ms = cas;
     }
     g->timing.levels = int64_t(s);
-    g->timing.launches = int64_t(s) * 2 + 3;
+    g->timing.binned = g->h_ctl->binned;
+    g->timing.launches = int64_t(s) * 2 + 3 + (g->bin_on ? 3 * int64_t(std::min<uint32_t>(s - 1, g->bin_steps)) : 0);
 
     g->h_count.assign(size_t(g->kcap) * L, 0);
     HIP_TRY(io.d2h(g->h_count.data(), g->d_ihist, sizeof(int64_t) * g->h_count.size()));

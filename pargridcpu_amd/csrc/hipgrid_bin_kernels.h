@@ -1,0 +1,643 @@
+// hipgrid_bin_kernels.h -- binned apply of a level's first removal round.
+//
+// The first round of a level (the vertices its scan removes, F0) carries
+// most of a decomposition's decrements: on the 100M-vertex workload a level's
+// F0 decrements 10-35 % of the alive vertices' degree words at once.  Applied
+// as scattered device-scope atomics (one 128-byte line request each, ~18 G/s
+// on a 3.2 GB array whether or not the old value is returned) they are bound
+// by the atomic request rate.  Here they become records (the target word
+// u*L + l), moved by streaming passes, and each target range is applied by
+// one workgroup in LDS:
+//   k_bin_expand   decides which instances bin this step (frontier entries
+//                  against alive words); expands the F0 chunks once, writes
+//                  every entry whose target is alive as a record (expansion
+//                  order) and counts the records per coarse bin (a coarse bin
+//                  is a range of vertices holding about D / C adjacency
+//                  entries, so the hub ranges are short);
+//   k_bin_scatter  streams the records into their coarse bins' regions (LDS
+//                  staging per bin, one global reservation per run);
+//   k_bin_apply    one workgroup per (instance, coarse bin): groups the
+//                  region's records by fine bin (vb vertices whose degree
+//                  words fit in LDS), counts them per degree word with LDS
+//                  atomics, applies each word's count to the row it owns and
+//                  updates cnt from the layers that fell below the level's
+//                  threshold: a vertex left with fewer than lmd layers at or
+//                  above it is removed (index k).  Its work is proportional
+//                  to the records, not to the vertices of the range.
+// The vertices removed by the apply (F1) go to the step's chunk list and the
+// cascade expands them as it would the frontier; from there the level runs
+// as before.  The result is the same: each degree word loses exactly one
+// per removed neighbour, and a vertex leaves at level k exactly when its
+// count of layers at or above k+1 falls below lmd, whichever round it falls
+// in.  cnt stays exact for every alive vertex (it loses exactly the layers
+// that crossed), and every removed vertex holds cnt < lmd, as the cascade's
+// crossing rule needs.
+#pragma once
+
+#include "hipgrid_kernels.h"
+
+namespace hg {
+namespace {
+
+constexpr uint32_t kBinMaxC = 1024;         // coarse bins (LDS tables of the count and scatter kernels)
+constexpr uint32_t kBinMaxF = kBlock;       // fine bins per coarse bin (one per thread in the prefix sum)
+constexpr uint32_t kBinStageWords = 11264;  // scatter staging: C * S words
+constexpr uint32_t kBinDeltaWords = 16384;  // apply: LDS decrement counters of one fine bin
+constexpr uint32_t kBinUnroll = 4;
+
+struct BinGeo {
+    const uint32_t* cbw;  // [C+1] coarse bin boundaries as word indices (vertex * L): 0 = cbw[0] < ... < cbw[C] = n*L
+    uint32_t* bh;         // [I][cmax] F0 entries per coarse bin (an upper bound of its records); zero between uses
+    uint32_t* bc;         // [I][cmax] records written per coarse bin; zero between uses
+    uint64_t* bb;         // [I][cmax] first record of each coarse bin's region
+    uint32_t* rec1;       // [rcap] records grouped by coarse bin
+    uint32_t* rec2;       // [rcap] records grouped by fine bin within each coarse bin's region
+    uint64_t rcap;
+    uint64_t min_ent;     // an instance bins a step only with at least this many frontier entries
+    const uint16_t* gsel; // [ng+1] lowest coarse bin overlapping word granule g (words g << gsh ...); gsel[ng] = C-1
+    uint32_t ng, gsh;
+    uint32_t C, cmax;
+    uint32_t vbw;         // degree words per fine bin (vertices per fine bin * L)
+    uint32_t stage;       // scatter staging slots per coarse bin (<= kWave)
+    uint32_t dens_num, dens_den;  // ... and frontier entries * den >= num * alive vertices * L
+};
+
+struct BinDecide {
+    int32_t mode[64];
+    unsigned long long rbase[64];
+};
+
+// Which instances bin this step, and their record slices (instance order,
+// while the buffers hold their frontier entries).  Every block computes the
+// same answer from words no kernel writes meanwhile.  All threads call.
+__device__ __forceinline__ void bin_decide(const InstState* states, uint32_t I, uint32_t par, const BinGeo& bg,
+                                           uint32_t L, BinDecide* d) {
+    if (threadIdx.x == 0) {
+        unsigned long long base = 0;
+        for (uint32_t i = 0; i < I; ++i) {
+            const InstState* st = &states[i];
+            int32_t mode = 0;
+            unsigned long long rb = 0;
+            if (st->dphase[par] == kPhaseScan) {
+                const unsigned long long ent = st->pfent[par][0];
+                const unsigned long long al = st->alive[0];
+                if (ent >= bg.min_ent && ent * bg.dens_den >= (unsigned long long)bg.dens_num * al * L &&
+                    base + ent <= bg.rcap) {
+                    mode = 1;
+                    rb = base;
+                    base += ent;
+                }
+            }
+            d->mode[i] = mode;
+            d->rbase[i] = rb;
+        }
+    }
+    __syncthreads();
+}
+
+constexpr uint32_t kBinMaxG = 8192;  // word granules of the coarse-bin lookup table
+
+// Coarse bin of target word e: the largest c < C with cb[c] <= e, searched
+// among the bins that overlap e's granule (one bin in the sparse ranges).
+__device__ __forceinline__ uint32_t coarse_of(const uint32_t* cb, const uint16_t* gs, uint32_t gsh, uint32_t e) {
+    const uint32_t g = e >> gsh;
+    uint32_t lo = gs[g], hi = gs[g + 1];
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (cb[mid] <= e) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// A block's batch of adjacency ranges (one per thread, length 0 for none):
+// exclusive prefix of the lengths and the starts.
+struct BatchShared {
+    uint32_t pre[kBlock];
+    uint64_t start[kBlock];
+    uint32_t tmp[kWavesPerBlock + 1];
+};
+
+// Fills the batch tables; returns the batch's entry count.  All threads call.
+__device__ __forceinline__ uint32_t batch_load(BatchShared* bs, uint64_t start, uint32_t len) {
+    uint32_t total;
+    const uint32_t pre = block_excl_scan(len, &total, bs->tmp);
+    bs->pre[threadIdx.x] = pre;
+    bs->start[threadIdx.x] = start;
+    __syncthreads();
+    return total;
+}
+
+// Adjacency position of entry j < total of the batch.
+__device__ __forceinline__ uint64_t batch_pos(const BatchShared* bs, uint32_t j) {
+    uint32_t lo = 0, hi = kBlock - 1;  // largest p with pre[p] <= j (its range is non-empty)
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (bs->pre[mid] <= j) lo = mid;
+        else hi = mid - 1;
+    }
+    return bs->start[lo] + (j - bs->pre[lo]);
+}
+
+// The frontier chunk this thread takes in round m0 of the chunks dealt
+// round robin over the grid (as the cascade deals them).
+__device__ __forceinline__ void frontier_chunk(const uint64_t* pl, uint64_t pc, uint64_t m0, uint64_t* start,
+                                               uint32_t* len) {
+    const uint64_t j = (m0 + threadIdx.x) * gridDim.x + blockIdx.x;
+    *start = 0;
+    *len = 0;
+    if (j < pc) {
+        const uint64_t it = pl[j];
+        *start = item_start(it);
+        *len = item_len(it);
+    }
+}
+
+constexpr uint32_t kBinOutCap = 4096;  // expand: records gathered in LDS before one reservation
+
+struct BinExpandShared {
+    BinDecide d;
+    uint32_t cb[kBinMaxC + 1];
+    uint16_t gs[kBinMaxG + 1];
+    uint32_t hist[kBinMaxC];
+    BatchShared bs;
+    uint32_t out[kBinOutCap];
+    uint32_t nout;
+    unsigned long long base;
+};
+
+// Writes the gathered records to the instance's slice (one reservation) and
+// empties the buffer.  All threads call.
+__device__ __forceinline__ void bin_out_flush(BinExpandShared* sh, InstState* st, Control* ctl, uint32_t par,
+                                              uint32_t* rec0, unsigned long long lim) {
+    const uint32_t n = sh->nout;
+    if (threadIdx.x == 0 && n) sh->base = atomicAdd(&st->r0cnt[par][0], (unsigned long long)n);
+    __syncthreads();
+    if (n) {
+        const unsigned long long b = sh->base;
+        if (b + n > lim) {  // cannot happen: records are a subset of the frontier's entries
+            if (threadIdx.x == 0) atomicCAS(&ctl->error, 0, int32_t(kErrBin));
+        } else {
+            for (uint32_t t = threadIdx.x; t < n; t += kBlock) rec0[b + t] = sh->out[t];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) sh->nout = 0;
+    __syncthreads();
+}
+
+// Decision, then per binned instance: the F0 entries whose target is alive
+// become records, written in expansion order to the instance's slice of the
+// second record buffer (one reservation per block and batch), and bh[i][c]
+// counts them per coarse bin.
+__global__ void __launch_bounds__(kBlock) k_bin_expand(const Inst* insts, InstState* states, Control* ctl,
+                                                       uint32_t I, uint32_t s, BinGeo bg) {
+    __shared__ BinExpandShared sh;
+    const uint32_t par = s & 1u;
+    for (uint32_t t = threadIdx.x; t <= bg.C; t += kBlock) sh.cb[t] = bg.cbw[t];
+    for (uint32_t t = threadIdx.x; t <= bg.ng; t += kBlock) sh.gs[t] = bg.gsel[t];
+    bin_decide(states, I, par, bg, insts[0].L, &sh.d);
+    if (blockIdx.x == 0 && threadIdx.x < I) {
+        states[threadIdx.x].bmode[par] = sh.d.mode[threadIdx.x];
+        states[threadIdx.x].rbase[par] = sh.d.rbase[threadIdx.x];
+        if (sh.d.mode[threadIdx.x]) atomicAdd(&ctl->binned, 1);
+    }
+    for (uint32_t i = 0; i < I; ++i) {
+        if (!sh.d.mode[i]) continue;
+        const Inst& in = insts[i];
+        InstState* st = &states[i];
+        const bool compact = (in.flags & kFlagCompact) != 0;
+        const uint64_t rb = sh.d.rbase[i];
+        const unsigned long long lim = st->pfent[par][0];  // the slice holds this many records
+        uint32_t* rec0 = bg.rec2 + rb;
+        for (uint32_t t = threadIdx.x; t < bg.C; t += kBlock) sh.hist[t] = 0;
+        if (threadIdx.x == 0) sh.nout = 0;
+        __syncthreads();
+        const uint64_t pc = st->pfront[par][0];
+        const uint64_t* pl = defer_list(in, par);
+        for (uint64_t m0 = 0; m0 * gridDim.x + blockIdx.x < pc; m0 += kBlock) {
+            uint64_t start;
+            uint32_t len;
+            frontier_chunk(pl, pc, m0, &start, &len);
+            const uint32_t total = batch_load(&sh.bs, start, len);
+            for (uint32_t j0 = 0; j0 < total; j0 += kBlock * kBinUnroll) {
+                uint32_t e[kBinUnroll];
+                bool ok[kBinUnroll];
+#pragma unroll
+                for (uint32_t r = 0; r < kBinUnroll; ++r) {
+                    const uint32_t j = j0 + r * kBlock + threadIdx.x;
+                    ok[r] = j < total;
+                    e[r] = 0;
+                    if (ok[r]) {
+                        const uint64_t p = batch_pos(&sh.bs, j);
+                        e[r] = in.adj[p];
+                        ok[r] = compact || in.adjz[p] >= in.lmd;
+                    }
+                }
+#pragma unroll
+                for (uint32_t r = 0; r < kBinUnroll; ++r) {
+                    if (ok[r]) ok[r] = is_alive(in, div_L(e[r], in.L, in.lmagic));
+                    if (ok[r]) atomicAdd(&sh.hist[coarse_of(sh.cb, sh.gs, bg.gsh, e[r])], 1u);
+                }
+                // gather in LDS (wave-aggregated slots)
+#pragma unroll
+                for (uint32_t r = 0; r < kBinUnroll; ++r) {
+                    const uint64_t m = __ballot(ok[r]);
+                    if (m == 0) continue;
+                    uint32_t at = 0;
+                    if (lane_id() == 0) at = atomicAdd(&sh.nout, uint32_t(__popcll(m)));
+                    at = __shfl(at, 0, kWave);
+                    if (ok[r])
+                        sh.out[at + __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u))] =
+                            e[r];
+                }
+                __syncthreads();
+                if (sh.nout > kBinOutCap - kBlock * kBinUnroll) bin_out_flush(&sh, st, ctl, par, rec0, lim);
+            }
+            __syncthreads();  // the batch tables are refilled next round
+        }
+        bin_out_flush(&sh, st, ctl, par, rec0, lim);
+        uint32_t* bh = bg.bh + uint64_t(i) * bg.cmax;
+        for (uint32_t t = threadIdx.x; t < bg.C; t += kBlock)
+            if (sh.hist[t]) atomicAdd(&bh[t], sh.hist[t]);
+        __syncthreads();
+    }
+}
+
+struct BinScatterShared {
+    int32_t mode[64];
+    unsigned long long rbase[64];
+    uint32_t cb[kBinMaxC + 1];
+    uint16_t gs[kBinMaxG + 1];
+    uint32_t boff[kBinMaxC];   // region offset of each coarse bin within the instance's slice
+    uint32_t fill[kBinMaxC];   // staging slots taken (may pass S while full; reset by the flush)
+    uint32_t stage[kBinStageWords];
+    uint16_t full[kBinMaxC];
+    uint32_t nfull;
+    uint32_t tmp[kWavesPerBlock + 1];
+    unsigned long long nbatch, niter, nflush;
+};
+
+// Writes out the staged records of every coarse bin holding at least
+// `thresh` of them (S: the full ones; 1: all), reserving each run with one
+// atomic on the bin's cursor, and empties those bins.  A run that would pass
+// the bin's count (cannot happen: the count pass counted these very records)
+// is dropped and flags the error.  All threads call.
+__device__ __forceinline__ void bin_flush(BinScatterShared* sh, uint32_t C, uint32_t S, uint32_t thresh,
+                                          uint32_t* bcur, const uint32_t* bcap, uint32_t* rec, uint64_t rb,
+                                          Control* ctl) {
+    for (uint32_t c = threadIdx.x; c < C; c += kBlock)
+        if (sh->fill[c] >= thresh) sh->full[atomicAdd(&sh->nfull, 1u)] = uint16_t(c);
+    __syncthreads();
+    const uint32_t nf = sh->nfull;
+    const uint32_t lane = lane_id();
+    if (threadIdx.x == 0) sh->nflush += nf;
+    // a wave takes 64 listed bins at once: their reservations go out together
+    // (one round trip), then the wave writes the runs one after another
+    for (uint32_t q0 = (threadIdx.x / kWave) * kWave; q0 < nf; q0 += kBlock) {
+        const uint32_t q = q0 + lane;
+        uint32_t c = 0, cnt = 0, base = 0;
+        if (q < nf) {
+            c = sh->full[q];
+            const uint32_t f = sh->fill[c];
+            cnt = f < S ? f : S;
+            base = atomicAdd(&bcur[c], cnt);
+            if (uint64_t(base) + cnt > bcap[c]) {
+                atomicCAS(&ctl->error, 0, int32_t(kErrBin));
+                cnt = 0;
+            }
+        }
+        const uint32_t nj = nf - q0 < uint32_t(kWave) ? nf - q0 : uint32_t(kWave);
+        for (uint32_t j = 0; j < nj; ++j) {
+            const uint32_t cj = __shfl(c, int(j), kWave), kj = __shfl(cnt, int(j), kWave);
+            const uint32_t bj = __shfl(base, int(j), kWave);
+            if (lane < kj) rec[rb + sh->boff[cj] + bj + lane] = sh->stage[cj * S + lane];
+        }
+    }
+    __syncthreads();
+    for (uint32_t q = threadIdx.x; q < nf; q += kBlock) sh->fill[sh->full[q]] = 0;
+    if (threadIdx.x == 0) sh->nfull = 0;
+    __syncthreads();
+}
+
+// Per binned instance: the records (expansion order) are streamed in equal
+// slices over the grid and moved into their coarse bins' regions of rec1.
+__global__ void __launch_bounds__(kBlock) k_bin_scatter(const Inst* insts, InstState* states, Control* ctl,
+                                                        uint32_t I, uint32_t s, BinGeo bg) {
+    __shared__ BinScatterShared sh;
+    const uint32_t par = s & 1u;
+    const uint32_t C = bg.C, S = bg.stage;
+    if (threadIdx.x < I) {
+        sh.mode[threadIdx.x] = states[threadIdx.x].bmode[par];
+        sh.rbase[threadIdx.x] = states[threadIdx.x].rbase[par];
+    }
+    for (uint32_t t = threadIdx.x; t <= C; t += kBlock) sh.cb[t] = bg.cbw[t];
+    for (uint32_t t = threadIdx.x; t <= bg.ng; t += kBlock) sh.gs[t] = bg.gsel[t];
+    if (threadIdx.x == 0) {
+        sh.nfull = 0;
+        sh.nbatch = sh.niter = sh.nflush = 0;
+    }
+    __syncthreads();
+    for (uint32_t i = 0; i < I; ++i) {
+        if (!sh.mode[i]) continue;
+        const uint32_t* bh = bg.bh + uint64_t(i) * bg.cmax;
+        uint32_t* bcur = bg.bc + uint64_t(i) * bg.cmax;
+        const uint64_t rb = sh.rbase[i];
+        // region offsets: exclusive prefix of the counts (block 0 publishes them)
+        uint32_t carry = 0;
+        for (uint32_t c0 = 0; c0 < C; c0 += kBlock) {
+            const uint32_t c = c0 + threadIdx.x;
+            const uint32_t x = c < C ? bh[c] : 0u;
+            uint32_t tot;
+            const uint32_t ex = block_excl_scan(x, &tot, sh.tmp);
+            if (c < C) {
+                sh.boff[c] = carry + ex;
+                if (blockIdx.x == 0) bg.bb[uint64_t(i) * bg.cmax + c] = rb + carry + ex;
+            }
+            carry += tot;
+        }
+        for (uint32_t t = threadIdx.x; t < C; t += kBlock) sh.fill[t] = 0;
+        __syncthreads();
+        const uint64_t R = states[i].r0cnt[par][0];
+        const uint64_t lo = R * blockIdx.x / gridDim.x, hi = R * (blockIdx.x + 1) / gridDim.x;
+        const uint32_t* rec0 = bg.rec2 + rb;
+        for (uint64_t t0 = lo; t0 < hi; t0 += kBlock * kBinUnroll) {
+            uint32_t e[kBinUnroll], cb[kBinUnroll];
+            bool pend[kBinUnroll];
+#pragma unroll
+            for (uint32_t r = 0; r < kBinUnroll; ++r) {
+                const uint64_t t = t0 + r * kBlock + threadIdx.x;
+                pend[r] = t < hi;
+                e[r] = pend[r] ? rec0[t] : 0u;
+            }
+#pragma unroll
+            for (uint32_t r = 0; r < kBinUnroll; ++r) cb[r] = pend[r] ? coarse_of(sh.cb, sh.gs, bg.gsh, e[r]) : 0u;
+            if (threadIdx.x == 0) sh.nbatch++;
+            // stage; bins that fill up are written out and the rest retried
+            for (;;) {
+                if (threadIdx.x == 0) sh.niter++;
+                bool left = false;
+#pragma unroll
+                for (uint32_t r = 0; r < kBinUnroll; ++r) {
+                    if (!pend[r]) continue;
+                    const uint32_t slot = atomicAdd(&sh.fill[cb[r]], 1u);
+                    if (slot < S) {
+                        sh.stage[cb[r] * S + slot] = e[r];
+                        pend[r] = false;
+                    } else {
+                        left = true;
+                    }
+                }
+                const int more = __syncthreads_or(left ? 1 : 0);
+                bin_flush(&sh, C, S, S, bcur, bh, bg.rec1, rb, ctl);
+                if (!more) break;
+            }
+        }
+        bin_flush(&sh, C, S, 1u, bcur, bh, bg.rec1, rb, ctl);
+    }
+    if (threadIdx.x == 0 && sh.nbatch) {
+        atomicAdd(&ctl->dbg[0], sh.nbatch);
+        atomicAdd(&ctl->dbg[1], sh.niter);
+        atomicAdd(&ctl->dbg[2], sh.nflush);
+    }
+}
+
+constexpr uint32_t kBinMaxVb = 4096;   // vertices per fine bin at most
+constexpr uint32_t kBinRemCap = 1024;  // removals listed in LDS before they are pushed
+
+struct BinApplyShared {
+    int32_t mode[64];
+    int32_t level[64];
+    uint32_t first[65];                 // tasks of the instances before i
+    __attribute__((aligned(16))) uint32_t delta[kBinDeltaWords];  // records per degree word of the fine bin (zero between uses)
+    uint32_t rem[kBinRemCap];
+    uint32_t nrem;
+    uint32_t fstart[kBinMaxF + 1];
+    uint32_t fcur[kBinMaxF];
+    uint32_t tmp[kWavesPerBlock + 1];
+    unsigned long long base;
+    uint32_t removed;
+};
+
+// Pushes the listed removals as chunks for the cascade.  All threads call.
+__device__ __forceinline__ void bin_push_removed(const Inst& in, InstState* st, Control* ctl, BinApplyShared* sh,
+                                                 uint32_t par) {
+    const uint32_t nr = sh->nrem;
+    for (uint32_t q0 = 0; q0 < nr; q0 += kBlock) {
+        const uint32_t q = q0 + threadIdx.x;
+        const bool has = q < nr;
+        block_defer_push(in, defer_list(in, par), &st->pbin[par][0], ctl, has, has ? sh->rem[q] : 0u, sh->tmp,
+                         &sh->base);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        sh->removed += nr;
+        sh->nrem = 0;
+    }
+    __syncthreads();
+}
+
+// The records [a, b) of one fine bin (degree words [fw0, fw0 + nw)): counted
+// per word in LDS, then a sweep over the bin's vertices: a vertex with any
+// count subtracts them from its row (the task owns these rows), clears them,
+// and recounts its layers at or above the threshold; with fewer than lmd it
+// is removed.  The counters are zero again afterwards.  All threads call.
+__device__ __forceinline__ void bin_apply_fine(const Inst& in, InstState* st, Control* ctl, BinApplyShared* sh,
+                                               const uint32_t* recf, bool agent, uint32_t a, uint32_t b,
+                                               uint32_t fw0, uint32_t nw, uint32_t par, int32_t k) {
+    const uint32_t L = in.L, thr = uint32_t(k) + 1u;
+    const uint32_t fv0 = fw0 / L, nv = nw / L;
+    // every record lies in [fw0, fw0 + nw) (the passes before placed it
+    // there); one that did not would be skipped, never index LDS out of range
+    for (uint32_t t0 = a + threadIdx.x; t0 < b; t0 += kBlock * kBinUnroll) {
+        uint32_t e[kBinUnroll];
+#pragma unroll
+        for (uint32_t r = 0; r < kBinUnroll; ++r) {
+            const uint32_t t = t0 + r * kBlock;
+            e[r] = t < b ? (agent ? ld_agent(&recf[t]) : recf[t]) : 0xFFFFFFFFu;
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < kBinUnroll; ++r)
+            if (e[r] - fw0 < nw) atomicAdd(&sh->delta[e[r] - fw0], 1u);
+    }
+    __syncthreads();
+    const bool quad = (L & 3u) == 0;
+    for (uint32_t ub = 0; ub < nv; ub += kBlock) {
+        const uint32_t uu = ub + threadIdx.x;
+        bool rem = false;
+        uint32_t u = 0;
+        if (uu < nv) {
+            uint32_t* dl = &sh->delta[uu * L];
+            uint32_t any = 0;
+            if (quad) {
+                for (uint32_t w = 0; w < L; w += 4) {
+                    const uint4 x = *reinterpret_cast<const uint4*>(dl + w);
+                    any |= x.x | x.y | x.z | x.w;
+                }
+            } else {
+                for (uint32_t l = 0; l < L; ++l) any |= dl[l];
+            }
+            if (any) {
+                u = fv0 + uu;
+                uint32_t* row = rec_of(in, u);
+                uint32_t c = 0;
+                if (quad) {
+                    for (uint32_t w = 0; w < L; w += 4) {
+                        uint4 x = *reinterpret_cast<const uint4*>(row + w);
+                        const uint4 d = *reinterpret_cast<const uint4*>(dl + w);
+                        // d <= x: each record is a removed neighbour the word still counts
+                        x.x -= d.x;
+                        x.y -= d.y;
+                        x.z -= d.z;
+                        x.w -= d.w;
+                        *reinterpret_cast<uint4*>(row + w) = x;
+                        *reinterpret_cast<uint4*>(dl + w) = make_uint4(0u, 0u, 0u, 0u);
+                        c += uint32_t(x.x >= thr) + uint32_t(x.y >= thr) + uint32_t(x.z >= thr) + uint32_t(x.w >= thr);
+                    }
+                } else {
+                    for (uint32_t l = 0; l < L; ++l) {
+                        const uint32_t x = row[l] - dl[l];
+                        row[l] = x;
+                        dl[l] = 0;
+                        c += x >= thr;
+                    }
+                }
+                *cnt_of(in, u) = int32_t(c);
+                if (c < in.lmd) {  // the vertex was alive (at least lmd layers) before these records
+                    rem = true;
+                    *idx_of(in, u) = k;
+                    clear_alive(in, u);
+                }
+            }
+        }
+        if (rem) sh->rem[atomicAdd(&sh->nrem, 1u)] = u;
+        __syncthreads();
+        if (sh->nrem > kBinRemCap - kBlock) bin_push_removed(in, st, ctl, sh, par);
+    }
+}
+
+// One (instance, coarse bin) task: the region's records grouped by fine bin
+// (unless the coarse bin is one fine bin), then each fine bin applied.  All
+// threads call.
+__device__ __forceinline__ void bin_apply_task(const Inst& in, InstState* st, Control* ctl, const BinGeo& bg,
+                                               BinApplyShared* sh, uint32_t i, uint32_t c, uint32_t par, int32_t k) {
+    const uint64_t q = uint64_t(i) * bg.cmax + c;
+    const uint32_t nrec = bg.bc[q];
+    const uint64_t base = bg.bb[q];
+    const uint32_t w0 = bg.cbw[c], w1 = bg.cbw[c + 1];
+    const uint32_t vbw = bg.vbw;
+    const uint32_t nf = (w1 - w0 + vbw - 1) / vbw;  // <= kBinMaxF (the host sizes coarse bins so)
+    if (nrec > 0 && nf == 1) {
+        bin_apply_fine(in, st, ctl, sh, bg.rec1 + base, false, 0, nrec, w0, w1 - w0, par, k);
+    } else if (nrec > 0) {
+        for (uint32_t f = threadIdx.x; f <= nf; f += kBlock) sh->fstart[f] = 0;
+        __syncthreads();
+        const uint32_t* r1 = bg.rec1 + base;
+        for (uint32_t t0 = threadIdx.x; t0 < nrec; t0 += kBlock * kBinUnroll) {
+            uint32_t e[kBinUnroll];
+#pragma unroll
+            for (uint32_t r = 0; r < kBinUnroll; ++r) {
+                const uint32_t t = t0 + r * kBlock;
+                e[r] = t < nrec ? r1[t] : 0xFFFFFFFFu;
+            }
+#pragma unroll
+            for (uint32_t r = 0; r < kBinUnroll; ++r) {
+                if (e[r] == 0xFFFFFFFFu) continue;
+                if (e[r] >= w0 && e[r] < w1) atomicAdd(&sh->fstart[(e[r] - w0) / vbw], 1u);
+                else atomicCAS(&ctl->error, 0, int32_t(kErrBin));  // cannot happen: the scatter binned it here
+            }
+        }
+        __syncthreads();
+        {
+            const uint32_t x = threadIdx.x < nf ? sh->fstart[threadIdx.x] : 0u;
+            uint32_t tot;
+            const uint32_t ex = block_excl_scan(x, &tot, sh->tmp);
+            if (threadIdx.x < nf) {
+                sh->fstart[threadIdx.x] = ex;
+                sh->fcur[threadIdx.x] = ex;
+            }
+            if (threadIdx.x == 0) sh->fstart[nf] = tot;
+        }
+        __syncthreads();
+        uint32_t* r2 = bg.rec2 + base;
+        for (uint32_t t0 = threadIdx.x; t0 < nrec; t0 += kBlock * kBinUnroll) {
+            uint32_t e[kBinUnroll];
+#pragma unroll
+            for (uint32_t r = 0; r < kBinUnroll; ++r) {
+                const uint32_t t = t0 + r * kBlock;
+                e[r] = t < nrec ? r1[t] : 0xFFFFFFFFu;
+            }
+#pragma unroll
+            for (uint32_t r = 0; r < kBinUnroll; ++r)
+                if (e[r] >= w0 && e[r] < w1) r2[atomicAdd(&sh->fcur[(e[r] - w0) / vbw], 1u)] = e[r];
+        }
+        sync_stores();  // rec2 is read back below by other waves of the block
+        for (uint32_t f = 0; f < nf; ++f) {
+            const uint32_t a = sh->fstart[f], b = sh->fstart[f + 1];
+            if (a == b) continue;  // no records: no row of this fine bin changes
+            const uint32_t fw0 = w0 + f * vbw;
+            bin_apply_fine(in, st, ctl, sh, r2, true, a, b, fw0, (w1 - fw0) < vbw ? (w1 - fw0) : vbw, par, k);
+        }
+    }
+    bin_push_removed(in, st, ctl, sh, par);
+    if (threadIdx.x == 0) {
+        bg.bh[q] = 0;
+        bg.bc[q] = 0;
+        if (nrec) atomicAdd(&st->brec[0], (unsigned long long)nrec);
+    }
+}
+
+// Tasks (instance, coarse bin) of the binned instances, dealt round robin
+// over the grid.
+__global__ void __launch_bounds__(kBlock) k_bin_apply(const Inst* insts, InstState* states, Control* ctl, uint32_t I,
+                                                      uint32_t s, BinGeo bg) {
+    __shared__ BinApplyShared sh;
+    const uint32_t par = s & 1u;
+    if (threadIdx.x < I) {
+        sh.mode[threadIdx.x] = states[threadIdx.x].bmode[par];
+        sh.level[threadIdx.x] = states[threadIdx.x].dk[par];
+    }
+    for (uint32_t t = threadIdx.x; t < kBinDeltaWords; t += kBlock) sh.delta[t] = 0;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (uint32_t i = 0; i < I; ++i) {
+            sh.first[i] = t;
+            if (sh.mode[i]) t += bg.C;
+        }
+        sh.first[I] = t;
+        sh.removed = 0;
+        sh.nrem = 0;
+    }
+    __syncthreads();
+    const uint32_t ntasks = sh.first[I];
+    for (uint32_t t = blockIdx.x; t < ntasks; t += gridDim.x) {
+        uint32_t i = 0;
+        while (!(sh.mode[i] && t < sh.first[i] + bg.C)) ++i;  // t < ntasks: some binned instance holds it
+        const uint32_t c = t - sh.first[i];
+        bin_apply_task(insts[i], &states[i], ctl, bg, &sh, i, c, par, sh.level[i]);
+        if (threadIdx.x == 0) {
+            if (sh.removed) atomicAdd(&states[i].alive[0], 0ull - (unsigned long long)sh.removed);
+            sh.removed = 0;
+        }
+        __syncthreads();
+    }
+}
+
+// Entry boundaries of the coarse bins: bnd[j] = the first vertex v with
+// off[v] >= j * E, j = 0..J (a binary search per j).
+__global__ void k_bin_bounds(const int64_t* off, uint32_t n, uint64_t E, uint32_t J, uint32_t* bnd) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j > J) return;
+    const int64_t want = int64_t(uint64_t(j) * E);
+    uint32_t lo = 0, hi = n;  // off[n] = D >= every want < D; want >= D gives n
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (off[mid] >= want) hi = mid;
+        else lo = mid + 1;
+    }
+    bnd[j] = lo;
+}
+
+}  // namespace
+}  // namespace hg

@@ -180,6 +180,20 @@ hipError_t dalloc(T** p, size_t count) {
     return e;
 }
 
+// Binned apply of dense removal rounds (k_bin_*): used by graphs of at least
+// HG_BIN_MIN directed entries (default 2^26) and fewer than 2^32; HG_BIN=2
+// turns it off.  Its two record buffers hold bin_rcap(D) 4-byte records each.
+inline bool bin_enabled(uint64_t D) {
+    const int64_t lo = env_int("HG_BIN_MIN", 1 << 26);
+    return env_int("HG_BIN", 1) == 1 && D >= uint64_t(lo) && D < (1ull << 32);
+}
+inline uint64_t bin_rcap(uint64_t D) { return std::min<uint64_t>(3ull << 29, 2 * D + (1ull << 20)); }
+// Device bytes of the binned apply (record buffers and per-bin tables) for a
+// graph of D entries and L layers.
+inline uint64_t bin_bytes(uint64_t D, uint64_t L) {
+    return bin_enabled(D) ? 8 * bin_rcap(D) + 16 * 1025 * L + 4 * 1025 : 0;
+}
+
 // Runs fn(begin, end) over [0, count) in contiguous ranges of at least
 // min_per_thread items on up to 16 host threads (the caller's included).
 template <typename F>

@@ -117,22 +117,36 @@ constexpr uint32_t kFlagCompact = 2;
 // (pfront) and the chunks the previous step's cascade deferred to it
 // (pdefer).  One chunk list per parity holds both kinds: a step either
 // continues with deferred chunks or scans, never both.
+// A scan step of an instance may apply its frontier's decrements binned by
+// target (k_bin_*): bmode says so, rbase is its slice of the record buffers,
+// pfent counts the frontier's entries (the scan adds them) and pbin the
+// chunks of the vertices the binned apply removed, which the cascade then
+// takes from the same list instead of the frontier.
 struct alignas(128) InstState {
     int32_t dphase[2];
     int32_t dk[2];
-    int32_t pad0[28];
+    int32_t bmode[2];
+    int32_t pad0a[2];
+    unsigned long long rbase[2];
+    int32_t pad0[20];
     unsigned long long pdefer[2][16];  // one line each
     unsigned long long pfront[2][16];
     unsigned long long alive[16];      // vertices not yet removed (decremented mod 2^64)
+    unsigned long long pfent[2][16];
+    unsigned long long pbin[2][16];
+    unsigned long long brec[16];       // records the binned applies took (cumulative; diagnostics)
+    unsigned long long r0cnt[2][16];   // records of the step's binned frontier in expansion order
 };
 
 struct alignas(128) Control {
     int32_t error;   // nonzero: device-side check failed
     int32_t active;  // instances not done at the last level kernel
-    int32_t pad[30];
+    int32_t binned;  // first rounds applied binned (k_bin_expand, block 0)
+    int32_t pad[29];
+    unsigned long long dbg[16];  // event counters of the binned kernels (diagnostics)
 };
 
-enum Err : int32_t { kErrNone = 0, kErrDefer = 2, kErrCompact = 3 };
+enum Err : int32_t { kErrNone = 0, kErrDefer = 2, kErrCompact = 3, kErrBin = 4 };
 
 // ----------------------------------------------------------------- helpers
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
@@ -454,6 +468,10 @@ __device__ __forceinline__ void step_decide(InstState* states, Control* ctl, uin
             st->dk[s & 1u] = k;
             st->pdefer[(s + 1u) & 1u][0] = 0;
             st->pfront[(s + 1u) & 1u][0] = 0;
+            st->pfent[(s + 1u) & 1u][0] = 0;
+            st->pbin[(s + 1u) & 1u][0] = 0;
+            st->r0cnt[(s + 1u) & 1u][0] = 0;
+            st->bmode[(s + 1u) & 1u] = 0;
         }
     }
     __syncthreads();
@@ -470,6 +488,7 @@ struct ScanShared {
     uint32_t removed;  // removals by this block (for the alive counter)
     uint32_t tmp[kWavesPerBlock + 1];
     unsigned long long base;
+    unsigned long long ent;  // adjacency entries of the block's removals
 };
 
 // Survivors are appended to the block's alive list (in place: positions
@@ -485,7 +504,14 @@ __device__ __forceinline__ void scan_emit(const Inst& in, InstState* st, Control
         if (keep) list[base + __builtin_amdgcn_mbcnt_hi(uint32_t(km >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(km), 0u))] = v;
     }
     const uint64_t rm = __ballot(rem);
-    if (rm && lane_id() == 0) atomicAdd(&sh->removed, uint32_t(__popcll(rm)));
+    if (rm) {
+        unsigned long long d = rem ? (unsigned long long)(in.off[v + 1] - in.off[v]) : 0ull;
+        d = wave_sum_u64(d);
+        if (lane_id() == 0) {
+            atomicAdd(&sh->removed, uint32_t(__popcll(rm)));
+            atomicAdd(&sh->ent, d);
+        }
+    }
     block_defer_push(in, defer_list(in, parity), &st->pfront[parity][0], ctl, rem, v, sh->tmp, &sh->base);
 }
 
@@ -785,6 +811,7 @@ __global__ void __launch_bounds__(kBlock) k_scan(const Inst* insts, InstState* s
         if (threadIdx.x == 0) {
             sh.keep = 0;
             sh.removed = 0;
+            sh.ent = 0;
         }
         __syncthreads();
         for (uint32_t base = 0; base < len; base += kBlock) {
@@ -811,6 +838,7 @@ __global__ void __launch_bounds__(kBlock) k_scan(const Inst* insts, InstState* s
         if (threadIdx.x == 0) {
             in.blk[2 * blockIdx.x] = sh.keep;
             if (sh.removed) atomicAdd(&st->alive[0], 0ull - (unsigned long long)sh.removed);
+            if (sh.ent) atomicAdd(&st->pfent[s & 1u][0], sh.ent);
         }
         __syncthreads();
     }
@@ -1054,7 +1082,9 @@ __global__ void __launch_bounds__(kBlock) k_cascade(const Inst* insts, InstState
         const InstState* st = &states[threadIdx.x];
         sh.phase[threadIdx.x] = st->dphase[par];
         sh.level[threadIdx.x] = st->dk[par];
-        sh.chunks[threadIdx.x] = st->pdefer[par][0] + st->pfront[par][0];  // one of them is zero
+        // one of pdefer / pfront is zero; after a binned apply the list holds
+        // the vertices it removed instead of the frontier
+        sh.chunks[threadIdx.x] = st->bmode[par] ? st->pbin[par][0] : st->pdefer[par][0] + st->pfront[par][0];
     }
     __syncthreads();
     if (threadIdx.x == 0) {

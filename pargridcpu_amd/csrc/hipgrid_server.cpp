@@ -110,7 +110,7 @@ int64_t engine_bytes(const Request& r) {
     const uint64_t graph = 8 * (n + 1) + 5 * D + 4 * n * L + 5 * n + 4 * n * L + D / 8 + 8;
     const uint64_t state = L * (4 * n * L + 8 * n + 8 * n + n / 8 + 16 * (D / 4096 + n + 1));
     const uint64_t dens = 8 * L * (r.maxdeg + 2) * 4;
-    return int64_t(graph + state + dens + setup);
+    return int64_t(graph + state + dens + setup + hgc::bin_bytes(D, L));
 }
 int64_t batch_bytes(const Request& r) {
     const uint64_t n = uint64_t(r.n), L = uint64_t(r.L), D = r.D;

@@ -46,7 +46,7 @@ class Wds(C.Structure):
 class Timing(C.Structure):
     _fields_ = [("decompose_ms", C.c_double), ("init_ms", C.c_double), ("scan_ms", C.c_double),
                 ("cascade_ms", C.c_double), ("densest_ms", C.c_double), ("levels", C.c_int64),
-                ("launches", C.c_int64)]
+                ("launches", C.c_int64), ("binned", C.c_int64)]
 
 
 class BatchGraph(C.Structure):

@@ -77,6 +77,20 @@ def test_c3_sample_against_oracle(n, capsys):
     check_oracle(mg, ci, say)
 
 
+def test_c3_sample_binned(monkeypatch, capsys):
+    """The 2M-vertex sample (46M entries, below the default binning size)
+    with the binned apply forced on for every dense first round: exact."""
+    monkeypatch.setenv("HG_BIN_MIN", "1")
+    mg = MultilayerGraph.generate(c3_sample(2_000_000))
+    ci, cl, kmax, t = decompose(mg)
+    with capsys.disabled():
+        print(f"case c3 sample n=2M binned: {t.binned} binned rounds, {t.decompose_ms:.1f} ms", flush=True)
+    assert t.binned > 0
+    check_structure(ci, cl, kmax)
+    ref = O.coreindex_parallel(mg.n, mg.layers(), threads=8)
+    assert np.array_equal(ci, ref)
+
+
 @pytest.mark.parametrize("heavy,defer", [(256, 256), (1024, 4096)])
 def test_c3_sample_deferral_sizes(monkeypatch, heavy, defer, capsys):
     """Hubs far above the deferral threshold split into many chunks of
@@ -124,7 +138,8 @@ def test_c3_full_size(capsys):
     say(f"case c3 full size: generated n={mg.n} L={mg.L} D={mg.directed_entries()} in "
         f"{time.perf_counter() - t0:.0f} s")
     ci, cl, kmax, t = decompose(mg)
-    say(f"  engine: {t.decompose_ms:.1f} ms, kmax={kmax.tolist()}")
+    say(f"  engine: {t.decompose_ms:.1f} ms, {t.binned} binned rounds, kmax={kmax.tolist()}")
+    assert t.binned > 0  # the default path bins the dense first rounds of this graph
     check_structure(ci, cl, kmax)
     say(f"  structure ok at {time.perf_counter() - t0:.0f} s; oracle (one lmd per host thread)")
     check_oracle(mg, ci, say)

@@ -299,3 +299,73 @@ def test_row_compaction(monkeypatch, n, L, m, seed, gamma):
                 assert np.array_equal(eng.core_index(), ref), f"repeat {rep}"
                 cl, kmax = eng.cells()
                 assert np.array_equal(kmax, ref.max(axis=1))
+
+
+def force_binning(monkeypatch, coarse=None):
+    """Every scan step with a non-empty frontier takes the binned apply
+    (HG_BIN_MIN=1: any graph; HG_BIN_MINENT=1, HG_BIN_DENS=0: any density)."""
+    monkeypatch.setenv("HG_BIN_MIN", "1")
+    monkeypatch.setenv("HG_BIN_MINENT", "1")
+    monkeypatch.setenv("HG_BIN_DENS", "0")
+    monkeypatch.setenv("HG_BIN_STEPS", "100000")
+    if coarse:
+        monkeypatch.setenv("HG_BIN_C", str(coarse))
+
+
+@pytest.mark.parametrize("n,L,m,seed,gamma,coarse", [(3000, 5, 12000, 41, 2.3, None), (20000, 8, 90000, 42, 2.1, 64),
+                                                     (60000, 4, 400000, 43, 2.2, 1), (500, 2, 20000, 44, 2.5, 8),
+                                                     (2000, 12, 2500, 45, 2.3, None), (200, 1, 600, 46, 2.3, 3),
+                                                     (300000, 32, 40000, 47, 2.2, 2)])
+def test_binned_first_rounds(monkeypatch, n, L, m, seed, gamma, coarse):
+    """The binned apply of a level's first round (records grouped by coarse
+    and fine target range, applied in LDS, touched rows recounted) forced on
+    every scan step: exact against the oracle, also on a repeat of the same
+    handle and with a small grid.  Coarse bins from one (HG_BIN_C=1: ranges
+    split only by the fine-bin limit; the 32-layer case has 262144 vertices
+    per coarse bin at most, so n = 300000 splits) to hundreds."""
+    force_binning(monkeypatch, coarse)
+    mg = MultilayerGraph.generate(small_spec(n, L, m, seed, gamma=gamma))
+    ref = O.coreindex(mg.n, mg.layers())
+    for blocks in (0, 5):
+        if blocks:
+            monkeypatch.setenv("HG_BLOCKS", str(blocks))
+        say(f"case binned n={n} L={L} m={m} seed={seed} coarse={coarse} blocks={blocks}")
+        with Engine.from_graph(mg) as eng:
+            for rep in range(2):
+                eng.decompose()
+                assert np.array_equal(eng.core_index(), ref), f"repeat {rep}"
+                cl, kmax = eng.cells()
+                assert np.array_equal(kmax, ref.max(axis=1))
+                t = eng.timing()
+                if int(ref.max()) >= 1:
+                    assert t.binned > 0
+
+
+def test_binned_with_deferral_and_compaction(monkeypatch):
+    """The binned apply beside tiny deferral thresholds (its removals become
+    many chunks) and beside row compaction (it reads the compacted rows)."""
+    force_binning(monkeypatch, 16)
+    mg = MultilayerGraph.generate(small_spec(20000, 6, 80000, 48, gamma=2.1))
+    ref = O.coreindex(mg.n, mg.layers())
+    for env in ({"HG_HEAVY": "4", "HG_DEFER": "256"}, {"HG_COMPACT": "2"}):
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
+        say(f"case binned with {env}")
+        with Engine.from_graph(mg) as eng:
+            eng.decompose()
+            assert np.array_equal(eng.core_index(), ref)
+            assert eng.timing().binned > 0
+        for k in env:
+            monkeypatch.delenv(k)
+
+
+def test_binned_off_matches(monkeypatch):
+    """HG_BIN=2 keeps every round on the atomic path (no binned rounds), with
+    the same result."""
+    force_binning(monkeypatch)
+    monkeypatch.setenv("HG_BIN", "2")
+    mg = MultilayerGraph.generate(small_spec(3000, 5, 12000, 49))
+    with Engine.from_graph(mg) as eng:
+        eng.decompose()
+        assert eng.timing().binned == 0
+        assert np.array_equal(eng.core_index(), O.coreindex(mg.n, mg.layers()))
