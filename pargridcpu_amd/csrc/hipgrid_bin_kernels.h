@@ -361,24 +361,25 @@ __global__ void __launch_bounds__(kBlock) k_bin_scatter(const Inst* insts, InstS
         const uint64_t R = states[i].r0cnt[par][0];
         const uint64_t lo = R * blockIdx.x / gridDim.x, hi = R * (blockIdx.x + 1) / gridDim.x;
         const uint32_t* rec0 = bg.rec2 + rb;
-        for (uint64_t t0 = lo; t0 < hi; t0 += kBlock * kBinUnroll) {
-            uint32_t e[kBinUnroll], cb[kBinUnroll];
-            bool pend[kBinUnroll];
+        constexpr uint32_t U = 2 * kBinUnroll;  // records per thread and batch
+        for (uint64_t t0 = lo; t0 < hi; t0 += kBlock * U) {
+            uint32_t e[U], cb[U];
+            bool pend[U];
 #pragma unroll
-            for (uint32_t r = 0; r < kBinUnroll; ++r) {
+            for (uint32_t r = 0; r < U; ++r) {
                 const uint64_t t = t0 + r * kBlock + threadIdx.x;
                 pend[r] = t < hi;
                 e[r] = pend[r] ? rec0[t] : 0u;
             }
 #pragma unroll
-            for (uint32_t r = 0; r < kBinUnroll; ++r) cb[r] = pend[r] ? coarse_of(sh.cb, sh.gs, bg.gsh, e[r]) : 0u;
+            for (uint32_t r = 0; r < U; ++r) cb[r] = pend[r] ? coarse_of(sh.cb, sh.gs, bg.gsh, e[r]) : 0u;
             if (threadIdx.x == 0) sh.nbatch++;
             // stage; bins that fill up are written out and the rest retried
             for (;;) {
                 if (threadIdx.x == 0) sh.niter++;
                 bool left = false;
 #pragma unroll
-                for (uint32_t r = 0; r < kBinUnroll; ++r) {
+                for (uint32_t r = 0; r < U; ++r) {
                     if (!pend[r]) continue;
                     const uint32_t slot = atomicAdd(&sh.fill[cb[r]], 1u);
                     if (slot < S) {
