@@ -129,6 +129,7 @@ struct hg_graph {
     // counts / cursors / region starts per instance, two record buffers
     bool bin_on = false;
     uint32_t bin_steps = 0;  // steps 1..bin_steps may bin
+    uint32_t bin_rounds = 0; // removal rounds 0..bin_rounds-1 of such a step may bin
     BinGeo bgeo{};
     uint32_t* bin_cbw = nullptr;
     uint32_t* bin_bh = nullptr;
@@ -264,6 +265,9 @@ int bin_setup(hg_graph* g, Scratch& tmp, hgc::Staging& up) {
     bg.dens_num = dv && *dv ? uint32_t(std::max(0, std::atoi(dv))) : 100u;
     bg.dens_den = 1000;
     g->bin_steps = uint32_t(env_int("HG_BIN_STEPS", 64));
+    // removal rounds per level that may bin: later rounds are sparser, and on
+    // the 100M-vertex workload one round is fastest (494 ms; 2: 498.5, 3: 500)
+    g->bin_rounds = uint32_t(env_int("HG_BIN_ROUNDS", 1));
     g->bin_on = true;
     return HG_OK;
 }
@@ -587,7 +591,7 @@ int launch_step(hg_graph* g, uint32_t s) {
     // diagnostic (HG_BIN_LOG=1): every step's kernels timed one by one and
     // the binned instances' frontier entries, records and removals printed
     static const bool blog = env_int("HG_BIN_LOG", 0) == 1;
-    double tk[4] = {0, 0, 0, 0};
+    double tk[4] = {0, 0, 0, 0}, tsum[3] = {0, 0, 0};
     auto tick = [&](int slot, std::chrono::steady_clock::time_point t0) -> int {
         HIP_TRY(hipStreamSynchronize(g->stream));
         tk[slot] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -603,18 +607,26 @@ int launch_step(hg_graph* g, uint32_t s) {
         // dense first rounds: binned apply (the kernels return at once for
         // instances that do not bin this step)
         const uint32_t g2 = 2u * uint32_t(g->cus);
-        auto t0 = std::chrono::steady_clock::now();
-        hipLaunchKernelGGL(k_bin_expand, dim3(g->G), dim3(kBlock), 0, g->stream, g->d_insts, g->d_states, g->d_ctl, L,
-                           s, g->bgeo);
-        if (blog && tick(0, t0)) return HG_ERR_HIP;
-        t0 = std::chrono::steady_clock::now();
-        hipLaunchKernelGGL(k_bin_scatter, dim3(g2), dim3(kBlock), 0, g->stream, g->d_insts, g->d_states, g->d_ctl, L,
-                           s, g->bgeo);
-        if (blog && tick(1, t0)) return HG_ERR_HIP;
-        t0 = std::chrono::steady_clock::now();
-        hipLaunchKernelGGL(k_bin_apply, dim3(g2), dim3(kBlock), 0, g->stream, g->d_insts, g->d_states, g->d_ctl, L,
-                           s, g->bgeo);
-        if (blog && tick(2, t0)) return HG_ERR_HIP;
+        for (uint32_t r = 0; r < g->bin_rounds; ++r) {  // removal rounds 0, 1, ... of the level
+            double t[3] = {0, 0, 0};
+            auto t0 = std::chrono::steady_clock::now();
+            hipLaunchKernelGGL(k_bin_expand, dim3(g->G), dim3(kBlock), 0, g->stream, g->d_insts, g->d_states, g->d_ctl,
+                               L, s, r, g->bgeo);
+            if (blog && tick(0, t0)) return HG_ERR_HIP;
+            t[0] = tk[0];
+            t0 = std::chrono::steady_clock::now();
+            hipLaunchKernelGGL(k_bin_scatter, dim3(g2), dim3(kBlock), 0, g->stream, g->d_insts, g->d_states,
+                               g->d_ctl, L, s, r, g->bgeo);
+            if (blog && tick(1, t0)) return HG_ERR_HIP;
+            t[1] = tk[1];
+            t0 = std::chrono::steady_clock::now();
+            hipLaunchKernelGGL(k_bin_apply, dim3(g2), dim3(kBlock), 0, g->stream, g->d_insts, g->d_states, g->d_ctl,
+                               L, s, r, g->bgeo);
+            if (blog && tick(2, t0)) return HG_ERR_HIP;
+            t[2] = tk[2];
+            for (int q = 0; q < 3; ++q) tsum[q] += t[q];
+        }
+        for (int q = 0; q < 3; ++q) tk[q] = tsum[q];
     }
     auto t3 = std::chrono::steady_clock::now();
     hipLaunchKernelGGL(k_cascade, dim3(g->G), dim3(kBlock), 0, g->stream, g->d_insts, g->d_states, g->d_ctl, L, s);
@@ -628,9 +640,10 @@ int launch_step(hg_graph* g, uint32_t s) {
             const InstState& a = after[i];
             if (a.dphase[p] == kPhaseDone) continue;
             char buf[256];
-            std::snprintf(buf, sizeof buf, " | l%u %s k%d f0ent %llu rec %llu f1ch %llu alive %llu", i + 1,
-                          a.dphase[p] == kPhaseScan ? (a.bmode[p] ? "B" : "S") : "C", a.dk[p], a.pfent[p][0],
-                          a.brec[0] - before[i].brec[0], a.bmode[p] ? a.pbin[p][0] : 0ull, a.alive[0]);
+            std::snprintf(buf, sizeof buf, " | l%u %s%d k%d f0ent %llu rec %llu lastch %llu alive %llu", i + 1,
+                          a.dphase[p] == kPhaseScan ? (a.bmode[p] ? "B" : "S") : "C", a.bmode[p] ? a.bround[p] : -1,
+                          a.dk[p], a.pfent[p][0],
+                          a.brec[0] - before[i].brec[0], a.bmode[p] ? a.pbr[a.bround[p] & 1][0] : 0ull, a.alive[0]);
             line += buf;
         }
         Control c{};
@@ -896,7 +909,8 @@ int decompose_impl(hg_graph* g) {
     }
     g->timing.levels = int64_t(s);
     g->timing.binned = g->h_ctl->binned;
-    g->timing.launches = int64_t(s) * 2 + 3 + (g->bin_on ? 3 * int64_t(std::min<uint32_t>(s - 1, g->bin_steps)) : 0);
+    g->timing.launches = int64_t(s) * 2 + 3 +
+                         (g->bin_on ? 3 * int64_t(g->bin_rounds) * int64_t(std::min<uint32_t>(s - 1, g->bin_steps)) : 0);
 
     g->h_count.assign(size_t(g->kcap) * L, 0);
     HIP_TRY(io.d2h(g->h_count.data(), g->d_ihist, sizeof(int64_t) * g->h_count.size()));

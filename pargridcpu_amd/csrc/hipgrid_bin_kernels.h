@@ -67,19 +67,33 @@ struct BinDecide {
     unsigned long long rbase[64];
 };
 
-// Which instances bin this step, and their record slices (instance order,
-// while the buffers hold their frontier entries).  Every block computes the
-// same answer from words no kernel writes meanwhile.  All threads call.
-__device__ __forceinline__ void bin_decide(const InstState* states, uint32_t I, uint32_t par, const BinGeo& bg,
-                                           uint32_t L, BinDecide* d) {
+// Removal round r's input: the scan's frontier (r = 0) or the vertices round
+// r - 1 removed, which exists when round r - 1 binned.
+__device__ __forceinline__ bool bin_input(const InstState* st, uint32_t par, uint32_t r, unsigned long long* ent,
+                                          unsigned long long* chunks) {
+    if (r == 0) {
+        *ent = st->pfent[par][0];
+        *chunks = st->pfront[par][0];
+        return st->dphase[par] == kPhaseScan;
+    }
+    *ent = st->pber[(r - 1) & 1][0];
+    *chunks = st->pbr[(r - 1) & 1][0];
+    return st->dphase[par] == kPhaseScan && st->bflag[(r - 1) & 1] != 0;
+}
+
+// Which instances bin round r of this step, and their record slices
+// (instance order, while the buffers hold their input's entries).  Every
+// block computes the same answer from words no kernel writes meanwhile.  All
+// threads call.
+__device__ __forceinline__ void bin_decide(const InstState* states, uint32_t I, uint32_t par, uint32_t r,
+                                           const BinGeo& bg, uint32_t L, BinDecide* d) {
     if (threadIdx.x == 0) {
         unsigned long long base = 0;
         for (uint32_t i = 0; i < I; ++i) {
             const InstState* st = &states[i];
             int32_t mode = 0;
-            unsigned long long rb = 0;
-            if (st->dphase[par] == kPhaseScan) {
-                const unsigned long long ent = st->pfent[par][0];
+            unsigned long long rb = 0, ent = 0, chunks = 0;
+            if (bin_input(st, par, r, &ent, &chunks)) {
                 const unsigned long long al = st->alive[0];
                 if (ent >= bg.min_ent && ent * bg.dens_den >= (unsigned long long)bg.dens_num * al * L &&
                     base + ent <= bg.rcap) {
@@ -168,10 +182,10 @@ struct BinExpandShared {
 
 // Writes the gathered records to the instance's slice (one reservation) and
 // empties the buffer.  All threads call.
-__device__ __forceinline__ void bin_out_flush(BinExpandShared* sh, InstState* st, Control* ctl, uint32_t par,
+__device__ __forceinline__ void bin_out_flush(BinExpandShared* sh, InstState* st, Control* ctl, uint32_t rp,
                                               uint32_t* rec0, unsigned long long lim) {
     const uint32_t n = sh->nout;
-    if (threadIdx.x == 0 && n) sh->base = atomicAdd(&st->r0cnt[par][0], (unsigned long long)n);
+    if (threadIdx.x == 0 && n) sh->base = atomicAdd(&st->r0cnt[rp][0], (unsigned long long)n);
     __syncthreads();
     if (n) {
         const unsigned long long b = sh->base;
@@ -191,16 +205,25 @@ __device__ __forceinline__ void bin_out_flush(BinExpandShared* sh, InstState* st
 // second record buffer (one reservation per block and batch), and bh[i][c]
 // counts them per coarse bin.
 __global__ void __launch_bounds__(kBlock) k_bin_expand(const Inst* insts, InstState* states, Control* ctl,
-                                                       uint32_t I, uint32_t s, BinGeo bg) {
+                                                       uint32_t I, uint32_t s, uint32_t round, BinGeo bg) {
     __shared__ BinExpandShared sh;
-    const uint32_t par = s & 1u;
+    const uint32_t par = s & 1u, rp = round & 1u;
     for (uint32_t t = threadIdx.x; t <= bg.C; t += kBlock) sh.cb[t] = bg.cbw[t];
     for (uint32_t t = threadIdx.x; t <= bg.ng; t += kBlock) sh.gs[t] = bg.gsel[t];
-    bin_decide(states, I, par, bg, insts[0].L, &sh.d);
+    bin_decide(states, I, par, round, bg, insts[0].L, &sh.d);
     if (blockIdx.x == 0 && threadIdx.x < I) {
-        states[threadIdx.x].bmode[par] = sh.d.mode[threadIdx.x];
-        states[threadIdx.x].rbase[par] = sh.d.rbase[threadIdx.x];
-        if (sh.d.mode[threadIdx.x]) atomicAdd(&ctl->binned, 1);
+        // the round's flag, and the counters its apply fills (read by the
+        // next round or the cascade; nothing reads them during this kernel)
+        InstState* st = &states[threadIdx.x];
+        st->bflag[rp] = sh.d.mode[threadIdx.x];
+        if (sh.d.mode[threadIdx.x]) {
+            st->bmode[par] = 1;
+            st->bround[par] = int32_t(round);
+            st->rbase[par] = sh.d.rbase[threadIdx.x];
+            st->pbr[rp][0] = 0;
+            st->pber[rp][0] = 0;
+            atomicAdd(&ctl->binned, 1);
+        }
     }
     for (uint32_t i = 0; i < I; ++i) {
         if (!sh.d.mode[i]) continue;
@@ -208,12 +231,12 @@ __global__ void __launch_bounds__(kBlock) k_bin_expand(const Inst* insts, InstSt
         InstState* st = &states[i];
         const bool compact = (in.flags & kFlagCompact) != 0;
         const uint64_t rb = sh.d.rbase[i];
-        const unsigned long long lim = st->pfent[par][0];  // the slice holds this many records
+        unsigned long long lim, pc;  // the slice holds lim records: the input's entries
+        bin_input(st, par, round, &lim, &pc);
         uint32_t* rec0 = bg.rec2 + rb;
         for (uint32_t t = threadIdx.x; t < bg.C; t += kBlock) sh.hist[t] = 0;
         if (threadIdx.x == 0) sh.nout = 0;
         __syncthreads();
-        const uint64_t pc = st->pfront[par][0];
         const uint64_t* pl = defer_list(in, par);
         for (uint64_t m0 = 0; m0 * gridDim.x + blockIdx.x < pc; m0 += kBlock) {
             uint64_t start;
@@ -252,11 +275,11 @@ __global__ void __launch_bounds__(kBlock) k_bin_expand(const Inst* insts, InstSt
                             e[r];
                 }
                 __syncthreads();
-                if (sh.nout > kBinOutCap - kBlock * kBinUnroll) bin_out_flush(&sh, st, ctl, par, rec0, lim);
+                if (sh.nout > kBinOutCap - kBlock * kBinUnroll) bin_out_flush(&sh, st, ctl, rp, rec0, lim);
             }
             __syncthreads();  // the batch tables are refilled next round
         }
-        bin_out_flush(&sh, st, ctl, par, rec0, lim);
+        bin_out_flush(&sh, st, ctl, rp, rec0, lim);
         uint32_t* bh = bg.bh + uint64_t(i) * bg.cmax;
         for (uint32_t t = threadIdx.x; t < bg.C; t += kBlock)
             if (sh.hist[t]) atomicAdd(&bh[t], sh.hist[t]);
@@ -323,12 +346,12 @@ __device__ __forceinline__ void bin_flush(BinScatterShared* sh, uint32_t C, uint
 // Per binned instance: the records (expansion order) are streamed in equal
 // slices over the grid and moved into their coarse bins' regions of rec1.
 __global__ void __launch_bounds__(kBlock) k_bin_scatter(const Inst* insts, InstState* states, Control* ctl,
-                                                        uint32_t I, uint32_t s, BinGeo bg) {
+                                                        uint32_t I, uint32_t s, uint32_t round, BinGeo bg) {
     __shared__ BinScatterShared sh;
     const uint32_t par = s & 1u;
     const uint32_t C = bg.C, S = bg.stage;
     if (threadIdx.x < I) {
-        sh.mode[threadIdx.x] = states[threadIdx.x].bmode[par];
+        sh.mode[threadIdx.x] = states[threadIdx.x].bflag[round & 1u];
         sh.rbase[threadIdx.x] = states[threadIdx.x].rbase[par];
     }
     for (uint32_t t = threadIdx.x; t <= C; t += kBlock) sh.cb[t] = bg.cbw[t];
@@ -358,7 +381,7 @@ __global__ void __launch_bounds__(kBlock) k_bin_scatter(const Inst* insts, InstS
         }
         for (uint32_t t = threadIdx.x; t < C; t += kBlock) sh.fill[t] = 0;
         __syncthreads();
-        const uint64_t R = states[i].r0cnt[par][0];
+        const uint64_t R = states[i].r0cnt[round & 1u][0];
         const uint64_t lo = R * blockIdx.x / gridDim.x, hi = R * (blockIdx.x + 1) / gridDim.x;
         const uint32_t* rec0 = bg.rec2 + rb;
         constexpr uint32_t U = 2 * kBinUnroll;  // records per thread and batch
@@ -416,19 +439,22 @@ struct BinApplyShared {
     uint32_t fstart[kBinMaxF + 1];
     uint32_t fcur[kBinMaxF];
     uint32_t tmp[kWavesPerBlock + 1];
+    uint32_t claim[kBinMaxVb / 32];  // vertices of the fine bin already updated (record-driven path; zero between uses)
     unsigned long long base;
+    unsigned long long ent;  // entries of the removals pushed by the current task
     uint32_t removed;
 };
 
 // Pushes the listed removals as chunks for the cascade.  All threads call.
 __device__ __forceinline__ void bin_push_removed(const Inst& in, InstState* st, Control* ctl, BinApplyShared* sh,
-                                                 uint32_t par) {
+                                                 uint32_t par, uint32_t rp) {
     const uint32_t nr = sh->nrem;
     for (uint32_t q0 = 0; q0 < nr; q0 += kBlock) {
         const uint32_t q = q0 + threadIdx.x;
         const bool has = q < nr;
-        block_defer_push(in, defer_list(in, par), &st->pbin[par][0], ctl, has, has ? sh->rem[q] : 0u, sh->tmp,
-                         &sh->base);
+        const uint32_t v = has ? sh->rem[q] : 0u;
+        if (has) atomicAdd(&sh->ent, (unsigned long long)(in.off[v + 1] - in.off[v]));
+        block_defer_push(in, defer_list(in, par), &st->pbr[rp][0], ctl, has, v, sh->tmp, &sh->base);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -438,6 +464,45 @@ __device__ __forceinline__ void bin_push_removed(const Inst& in, InstState* st, 
     __syncthreads();
 }
 
+// Vertex uu of the fine bin starting at vertex fv0: subtracts its record
+// counts from its row (the task owns the row), clears them, stores the new
+// count of layers at or above thr and removes the vertex (index k) when
+// fewer than lmd are left; returns whether it did, with the vertex in *u.
+__device__ __forceinline__ bool bin_row_update(const Inst& in, BinApplyShared* sh, uint32_t uu, uint32_t fv0,
+                                               uint32_t thr, int32_t k, bool quad, uint32_t* u) {
+    const uint32_t L = in.L;
+    uint32_t* dl = &sh->delta[uu * L];
+    *u = fv0 + uu;
+    uint32_t* row = rec_of(in, *u);
+    uint32_t c = 0;
+    if (quad) {
+        for (uint32_t w = 0; w < L; w += 4) {
+            uint4 x = *reinterpret_cast<const uint4*>(row + w);
+            const uint4 d = *reinterpret_cast<const uint4*>(dl + w);
+            // d <= x: each record is a removed neighbour the word still counts
+            x.x -= d.x;
+            x.y -= d.y;
+            x.z -= d.z;
+            x.w -= d.w;
+            *reinterpret_cast<uint4*>(row + w) = x;
+            *reinterpret_cast<uint4*>(dl + w) = make_uint4(0u, 0u, 0u, 0u);
+            c += uint32_t(x.x >= thr) + uint32_t(x.y >= thr) + uint32_t(x.z >= thr) + uint32_t(x.w >= thr);
+        }
+    } else {
+        for (uint32_t l = 0; l < L; ++l) {
+            const uint32_t x = row[l] - dl[l];
+            row[l] = x;
+            dl[l] = 0;
+            c += x >= thr;
+        }
+    }
+    *cnt_of(in, *u) = int32_t(c);
+    if (c >= in.lmd) return false;  // the vertex was alive (at least lmd layers) before these records
+    *idx_of(in, *u) = k;
+    clear_alive(in, *u);
+    return true;
+}
+
 // The records [a, b) of one fine bin (degree words [fw0, fw0 + nw)): counted
 // per word in LDS, then a sweep over the bin's vertices: a vertex with any
 // count subtracts them from its row (the task owns these rows), clears them,
@@ -445,7 +510,7 @@ __device__ __forceinline__ void bin_push_removed(const Inst& in, InstState* st, 
 // is removed.  The counters are zero again afterwards.  All threads call.
 __device__ __forceinline__ void bin_apply_fine(const Inst& in, InstState* st, Control* ctl, BinApplyShared* sh,
                                                const uint32_t* recf, bool agent, uint32_t a, uint32_t b,
-                                               uint32_t fw0, uint32_t nw, uint32_t par, int32_t k) {
+                                               uint32_t fw0, uint32_t nw, uint32_t par, uint32_t rp, int32_t k) {
     const uint32_t L = in.L, thr = uint32_t(k) + 1u;
     const uint32_t fv0 = fw0 / L, nv = nw / L;
     // every record lies in [fw0, fw0 + nw) (the passes before placed it
@@ -463,12 +528,33 @@ __device__ __forceinline__ void bin_apply_fine(const Inst& in, InstState* st, Co
     }
     __syncthreads();
     const bool quad = (L & 3u) == 0;
+    if (uint64_t(b - a) * 4 < nv) {
+        // few records: each vertex is claimed by its first record (one LDS
+        // bit) and updated once; work follows the records, not the vertices
+        for (uint32_t t0 = a; t0 < b; t0 += kBlock) {
+            const uint32_t t = t0 + threadIdx.x;
+            bool rem = false;
+            uint32_t u = 0;
+            const uint32_t e = t < b ? (agent ? ld_agent(&recf[t]) : recf[t]) : 0xFFFFFFFFu;
+            if (e - fw0 < nw) {
+                const uint32_t uu = div_L(e - fw0, L, in.lmagic);
+                const uint32_t bit = 1u << (uu & 31u);
+                if ((atomicOr(&sh->claim[uu >> 5], bit) & bit) == 0) rem = bin_row_update(in, sh, uu, fv0, thr, k, quad, &u);
+            }
+            if (rem) sh->rem[atomicAdd(&sh->nrem, 1u)] = u;
+            __syncthreads();
+            if (sh->nrem > kBinRemCap - kBlock) bin_push_removed(in, st, ctl, sh, par, rp);
+        }
+        for (uint32_t w = threadIdx.x; w < (nv + 31) / 32; w += kBlock) sh->claim[w] = 0;
+        __syncthreads();
+        return;
+    }
     for (uint32_t ub = 0; ub < nv; ub += kBlock) {
         const uint32_t uu = ub + threadIdx.x;
         bool rem = false;
         uint32_t u = 0;
         if (uu < nv) {
-            uint32_t* dl = &sh->delta[uu * L];
+            const uint32_t* dl = &sh->delta[uu * L];
             uint32_t any = 0;
             if (quad) {
                 for (uint32_t w = 0; w < L; w += 4) {
@@ -478,42 +564,11 @@ __device__ __forceinline__ void bin_apply_fine(const Inst& in, InstState* st, Co
             } else {
                 for (uint32_t l = 0; l < L; ++l) any |= dl[l];
             }
-            if (any) {
-                u = fv0 + uu;
-                uint32_t* row = rec_of(in, u);
-                uint32_t c = 0;
-                if (quad) {
-                    for (uint32_t w = 0; w < L; w += 4) {
-                        uint4 x = *reinterpret_cast<const uint4*>(row + w);
-                        const uint4 d = *reinterpret_cast<const uint4*>(dl + w);
-                        // d <= x: each record is a removed neighbour the word still counts
-                        x.x -= d.x;
-                        x.y -= d.y;
-                        x.z -= d.z;
-                        x.w -= d.w;
-                        *reinterpret_cast<uint4*>(row + w) = x;
-                        *reinterpret_cast<uint4*>(dl + w) = make_uint4(0u, 0u, 0u, 0u);
-                        c += uint32_t(x.x >= thr) + uint32_t(x.y >= thr) + uint32_t(x.z >= thr) + uint32_t(x.w >= thr);
-                    }
-                } else {
-                    for (uint32_t l = 0; l < L; ++l) {
-                        const uint32_t x = row[l] - dl[l];
-                        row[l] = x;
-                        dl[l] = 0;
-                        c += x >= thr;
-                    }
-                }
-                *cnt_of(in, u) = int32_t(c);
-                if (c < in.lmd) {  // the vertex was alive (at least lmd layers) before these records
-                    rem = true;
-                    *idx_of(in, u) = k;
-                    clear_alive(in, u);
-                }
-            }
+            if (any) rem = bin_row_update(in, sh, uu, fv0, thr, k, quad, &u);
         }
         if (rem) sh->rem[atomicAdd(&sh->nrem, 1u)] = u;
         __syncthreads();
-        if (sh->nrem > kBinRemCap - kBlock) bin_push_removed(in, st, ctl, sh, par);
+        if (sh->nrem > kBinRemCap - kBlock) bin_push_removed(in, st, ctl, sh, par, rp);
     }
 }
 
@@ -521,7 +576,8 @@ __device__ __forceinline__ void bin_apply_fine(const Inst& in, InstState* st, Co
 // (unless the coarse bin is one fine bin), then each fine bin applied.  All
 // threads call.
 __device__ __forceinline__ void bin_apply_task(const Inst& in, InstState* st, Control* ctl, const BinGeo& bg,
-                                               BinApplyShared* sh, uint32_t i, uint32_t c, uint32_t par, int32_t k) {
+                                               BinApplyShared* sh, uint32_t i, uint32_t c, uint32_t par, uint32_t rp,
+                                               int32_t k) {
     const uint64_t q = uint64_t(i) * bg.cmax + c;
     const uint32_t nrec = bg.bc[q];
     const uint64_t base = bg.bb[q];
@@ -529,7 +585,7 @@ __device__ __forceinline__ void bin_apply_task(const Inst& in, InstState* st, Co
     const uint32_t vbw = bg.vbw;
     const uint32_t nf = (w1 - w0 + vbw - 1) / vbw;  // <= kBinMaxF (the host sizes coarse bins so)
     if (nrec > 0 && nf == 1) {
-        bin_apply_fine(in, st, ctl, sh, bg.rec1 + base, false, 0, nrec, w0, w1 - w0, par, k);
+        bin_apply_fine(in, st, ctl, sh, bg.rec1 + base, false, 0, nrec, w0, w1 - w0, par, rp, k);
     } else if (nrec > 0) {
         for (uint32_t f = threadIdx.x; f <= nf; f += kBlock) sh->fstart[f] = 0;
         __syncthreads();
@@ -577,10 +633,10 @@ __device__ __forceinline__ void bin_apply_task(const Inst& in, InstState* st, Co
             const uint32_t a = sh->fstart[f], b = sh->fstart[f + 1];
             if (a == b) continue;  // no records: no row of this fine bin changes
             const uint32_t fw0 = w0 + f * vbw;
-            bin_apply_fine(in, st, ctl, sh, r2, true, a, b, fw0, (w1 - fw0) < vbw ? (w1 - fw0) : vbw, par, k);
+            bin_apply_fine(in, st, ctl, sh, r2, true, a, b, fw0, (w1 - fw0) < vbw ? (w1 - fw0) : vbw, par, rp, k);
         }
     }
-    bin_push_removed(in, st, ctl, sh, par);
+    bin_push_removed(in, st, ctl, sh, par, rp);
     if (threadIdx.x == 0) {
         bg.bh[q] = 0;
         bg.bc[q] = 0;
@@ -591,14 +647,18 @@ __device__ __forceinline__ void bin_apply_task(const Inst& in, InstState* st, Co
 // Tasks (instance, coarse bin) of the binned instances, dealt round robin
 // over the grid.
 __global__ void __launch_bounds__(kBlock) k_bin_apply(const Inst* insts, InstState* states, Control* ctl, uint32_t I,
-                                                      uint32_t s, BinGeo bg) {
+                                                      uint32_t s, uint32_t round, BinGeo bg) {
     __shared__ BinApplyShared sh;
-    const uint32_t par = s & 1u;
+    const uint32_t par = s & 1u, rp = round & 1u;
     if (threadIdx.x < I) {
-        sh.mode[threadIdx.x] = states[threadIdx.x].bmode[par];
+        sh.mode[threadIdx.x] = states[threadIdx.x].bflag[rp];
         sh.level[threadIdx.x] = states[threadIdx.x].dk[par];
     }
     for (uint32_t t = threadIdx.x; t < kBinDeltaWords; t += kBlock) sh.delta[t] = 0;
+    for (uint32_t t = threadIdx.x; t < kBinMaxVb / 32; t += kBlock) sh.claim[t] = 0;
+    // the round's record count was read by the scatter; the next use of this
+    // slot (two rounds on) counts from zero
+    if (blockIdx.x == 0 && threadIdx.x < I && sh.mode[threadIdx.x]) states[threadIdx.x].r0cnt[rp][0] = 0;
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t t = 0;
@@ -608,6 +668,7 @@ __global__ void __launch_bounds__(kBlock) k_bin_apply(const Inst* insts, InstSta
         }
         sh.first[I] = t;
         sh.removed = 0;
+        sh.ent = 0;
         sh.nrem = 0;
     }
     __syncthreads();
@@ -616,10 +677,12 @@ __global__ void __launch_bounds__(kBlock) k_bin_apply(const Inst* insts, InstSta
         uint32_t i = 0;
         while (!(sh.mode[i] && t < sh.first[i] + bg.C)) ++i;  // t < ntasks: some binned instance holds it
         const uint32_t c = t - sh.first[i];
-        bin_apply_task(insts[i], &states[i], ctl, bg, &sh, i, c, par, sh.level[i]);
+        bin_apply_task(insts[i], &states[i], ctl, bg, &sh, i, c, par, rp, sh.level[i]);
         if (threadIdx.x == 0) {
             if (sh.removed) atomicAdd(&states[i].alive[0], 0ull - (unsigned long long)sh.removed);
+            if (sh.ent) atomicAdd(&states[i].pber[rp][0], sh.ent);
             sh.removed = 0;
+            sh.ent = 0;
         }
         __syncthreads();
     }

@@ -117,31 +117,37 @@ constexpr uint32_t kFlagCompact = 2;
 // (pfront) and the chunks the previous step's cascade deferred to it
 // (pdefer).  One chunk list per parity holds both kinds: a step either
 // continues with deferred chunks or scans, never both.
-// A scan step of an instance may apply its frontier's decrements binned by
-// target (k_bin_*): bmode says so, rbase is its slice of the record buffers,
-// pfent counts the frontier's entries (the scan adds them) and pbin the
-// chunks of the vertices the binned apply removed, which the cascade then
-// takes from the same list instead of the frontier.
+// A scan step of an instance may apply its first removal rounds binned by
+// target (k_bin_*, rounds r = 0, 1, ...: round 0 takes the scan's frontier,
+// round r > 0 the vertices round r - 1 removed).  bflag[r & 1] says whether
+// round r binned, bmode whether any round of the step did and bround the
+// last one; rbase is the round's slice of the record buffers; pfent counts
+// the frontier's entries (the scan adds them); pbr[r & 1] / pber[r & 1]
+// count the chunks and entries of the vertices round r removed, which the
+// next round or the cascade takes from the same list instead of the
+// frontier.
 struct alignas(128) InstState {
     int32_t dphase[2];
     int32_t dk[2];
     int32_t bmode[2];
-    int32_t pad0a[2];
+    int32_t bround[2];
     unsigned long long rbase[2];
-    int32_t pad0[20];
+    int32_t bflag[2];
+    int32_t pad0[18];
     unsigned long long pdefer[2][16];  // one line each
     unsigned long long pfront[2][16];
     unsigned long long alive[16];      // vertices not yet removed (decremented mod 2^64)
     unsigned long long pfent[2][16];
-    unsigned long long pbin[2][16];
+    unsigned long long pbr[2][16];
+    unsigned long long pber[2][16];
     unsigned long long brec[16];       // records the binned applies took (cumulative; diagnostics)
-    unsigned long long r0cnt[2][16];   // records of the step's binned frontier in expansion order
+    unsigned long long r0cnt[2][16];   // records of binned round r (slot r & 1) in expansion order; the round's apply clears it
 };
 
 struct alignas(128) Control {
     int32_t error;   // nonzero: device-side check failed
     int32_t active;  // instances not done at the last level kernel
-    int32_t binned;  // first rounds applied binned (k_bin_expand, block 0)
+    int32_t binned;  // (instance, round) pairs applied binned (k_bin_expand, block 0)
     int32_t pad[29];
     unsigned long long dbg[16];  // event counters of the binned kernels (diagnostics)
 };
@@ -469,8 +475,6 @@ __device__ __forceinline__ void step_decide(InstState* states, Control* ctl, uin
             st->pdefer[(s + 1u) & 1u][0] = 0;
             st->pfront[(s + 1u) & 1u][0] = 0;
             st->pfent[(s + 1u) & 1u][0] = 0;
-            st->pbin[(s + 1u) & 1u][0] = 0;
-            st->r0cnt[(s + 1u) & 1u][0] = 0;
             st->bmode[(s + 1u) & 1u] = 0;
         }
     }
@@ -1084,7 +1088,8 @@ __global__ void __launch_bounds__(kBlock) k_cascade(const Inst* insts, InstState
         sh.level[threadIdx.x] = st->dk[par];
         // one of pdefer / pfront is zero; after a binned apply the list holds
         // the vertices it removed instead of the frontier
-        sh.chunks[threadIdx.x] = st->bmode[par] ? st->pbin[par][0] : st->pdefer[par][0] + st->pfront[par][0];
+        sh.chunks[threadIdx.x] =
+            st->bmode[par] ? st->pbr[st->bround[par] & 1][0] : st->pdefer[par][0] + st->pfront[par][0];
     }
     __syncthreads();
     if (threadIdx.x == 0) {

@@ -359,6 +359,23 @@ def test_binned_with_deferral_and_compaction(monkeypatch):
             monkeypatch.delenv(k)
 
 
+@pytest.mark.parametrize("rounds", [1, 8])
+def test_binned_rounds(monkeypatch, rounds):
+    """Removal rounds after the first are binned too (round r takes the
+    vertices round r - 1 removed) up to HG_BIN_ROUNDS per level; one round
+    only, or up to eight: the same result."""
+    force_binning(monkeypatch, 32)
+    monkeypatch.setenv("HG_BIN_ROUNDS", str(rounds))
+    mg = MultilayerGraph.generate(small_spec(20000, 6, 80000, 50, gamma=2.1))
+    ref = O.coreindex(mg.n, mg.layers())
+    say(f"case binned rounds={rounds}")
+    with Engine.from_graph(mg) as eng:
+        for rep in range(2):
+            eng.decompose()
+            assert np.array_equal(eng.core_index(), ref), f"repeat {rep}"
+            assert eng.timing().binned > 0
+
+
 def test_binned_off_matches(monkeypatch):
     """HG_BIN=2 keeps every round on the atomic path (no binned rounds), with
     the same result."""
