@@ -130,6 +130,7 @@ struct hg_graph {
     bool bin_on = false;
     uint32_t bin_steps = 0;  // steps 1..bin_steps may bin
     uint32_t bin_rounds = 0; // removal rounds 0..bin_rounds-1 of such a step may bin
+    uint32_t bin_eu = 4;     // entries per thread in flight in k_bin_expand (4 or 8)
     BinGeo bgeo{};
     uint32_t* bin_cbw = nullptr;
     uint32_t* bin_bh = nullptr;
@@ -268,6 +269,7 @@ int bin_setup(hg_graph* g, Scratch& tmp, hgc::Staging& up) {
     // removal rounds per level that may bin: later rounds are sparser, and on
     // the 100M-vertex workload one round is fastest (494 ms; 2: 498.5, 3: 500)
     g->bin_rounds = uint32_t(env_int("HG_BIN_ROUNDS", 1));
+    g->bin_eu = env_int("HG_BIN_EU", 4) == 8 ? 8u : 4u;
     g->bin_on = true;
     return HG_OK;
 }
@@ -610,8 +612,12 @@ int launch_step(hg_graph* g, uint32_t s) {
         for (uint32_t r = 0; r < g->bin_rounds; ++r) {  // removal rounds 0, 1, ... of the level
             double t[3] = {0, 0, 0};
             auto t0 = std::chrono::steady_clock::now();
-            hipLaunchKernelGGL(k_bin_expand, dim3(g->G), dim3(kBlock), 0, g->stream, g->d_insts, g->d_states, g->d_ctl,
-                               L, s, r, g->bgeo);
+            if (g->bin_eu == 8)
+                hipLaunchKernelGGL(k_bin_expand<8>, dim3(g->G), dim3(kBlock), 0, g->stream, g->d_insts, g->d_states,
+                                   g->d_ctl, L, s, r, g->bgeo);
+            else
+                hipLaunchKernelGGL(k_bin_expand<4>, dim3(g->G), dim3(kBlock), 0, g->stream, g->d_insts, g->d_states,
+                                   g->d_ctl, L, s, r, g->bgeo);
             if (blog && tick(0, t0)) return HG_ERR_HIP;
             t[0] = tk[0];
             t0 = std::chrono::steady_clock::now();

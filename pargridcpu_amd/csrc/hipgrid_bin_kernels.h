@@ -167,22 +167,25 @@ __device__ __forceinline__ void frontier_chunk(const uint64_t* pl, uint64_t pc, 
     }
 }
 
-constexpr uint32_t kBinOutCap = 4096;  // expand: records gathered in LDS before one reservation
-
+// Expand: EU entries per thread in flight per round; records gathered in LDS
+// (OUTCAP = twice a round's) before one reservation.
+template <uint32_t EU>
 struct BinExpandShared {
+    static constexpr uint32_t kOutCap = 2 * kBlock * EU;
     BinDecide d;
     uint32_t cb[kBinMaxC + 1];
     uint16_t gs[kBinMaxG + 1];
     uint32_t hist[kBinMaxC];
     BatchShared bs;
-    uint32_t out[kBinOutCap];
+    uint32_t out[kOutCap];
     uint32_t nout;
     unsigned long long base;
 };
 
 // Writes the gathered records to the instance's slice (one reservation) and
 // empties the buffer.  All threads call.
-__device__ __forceinline__ void bin_out_flush(BinExpandShared* sh, InstState* st, Control* ctl, uint32_t rp,
+template <uint32_t EU>
+__device__ __forceinline__ void bin_out_flush(BinExpandShared<EU>* sh, InstState* st, Control* ctl, uint32_t rp,
                                               uint32_t* rec0, unsigned long long lim) {
     const uint32_t n = sh->nout;
     if (threadIdx.x == 0 && n) sh->base = atomicAdd(&st->r0cnt[rp][0], (unsigned long long)n);
@@ -204,9 +207,10 @@ __device__ __forceinline__ void bin_out_flush(BinExpandShared* sh, InstState* st
 // become records, written in expansion order to the instance's slice of the
 // second record buffer (one reservation per block and batch), and bh[i][c]
 // counts them per coarse bin.
+template <uint32_t EU>
 __global__ void __launch_bounds__(kBlock) k_bin_expand(const Inst* insts, InstState* states, Control* ctl,
                                                        uint32_t I, uint32_t s, uint32_t round, BinGeo bg) {
-    __shared__ BinExpandShared sh;
+    __shared__ BinExpandShared<EU> sh;
     const uint32_t par = s & 1u, rp = round & 1u;
     for (uint32_t t = threadIdx.x; t <= bg.C; t += kBlock) sh.cb[t] = bg.cbw[t];
     for (uint32_t t = threadIdx.x; t <= bg.ng; t += kBlock) sh.gs[t] = bg.gsel[t];
@@ -243,11 +247,11 @@ __global__ void __launch_bounds__(kBlock) k_bin_expand(const Inst* insts, InstSt
             uint32_t len;
             frontier_chunk(pl, pc, m0, &start, &len);
             const uint32_t total = batch_load(&sh.bs, start, len);
-            for (uint32_t j0 = 0; j0 < total; j0 += kBlock * kBinUnroll) {
-                uint32_t e[kBinUnroll];
-                bool ok[kBinUnroll];
+            for (uint32_t j0 = 0; j0 < total; j0 += kBlock * EU) {
+                uint32_t e[EU];
+                bool ok[EU];
 #pragma unroll
-                for (uint32_t r = 0; r < kBinUnroll; ++r) {
+                for (uint32_t r = 0; r < EU; ++r) {
                     const uint32_t j = j0 + r * kBlock + threadIdx.x;
                     ok[r] = j < total;
                     e[r] = 0;
@@ -258,13 +262,13 @@ __global__ void __launch_bounds__(kBlock) k_bin_expand(const Inst* insts, InstSt
                     }
                 }
 #pragma unroll
-                for (uint32_t r = 0; r < kBinUnroll; ++r) {
+                for (uint32_t r = 0; r < EU; ++r) {
                     if (ok[r]) ok[r] = is_alive(in, div_L(e[r], in.L, in.lmagic));
                     if (ok[r]) atomicAdd(&sh.hist[coarse_of(sh.cb, sh.gs, bg.gsh, e[r])], 1u);
                 }
                 // gather in LDS (wave-aggregated slots)
 #pragma unroll
-                for (uint32_t r = 0; r < kBinUnroll; ++r) {
+                for (uint32_t r = 0; r < EU; ++r) {
                     const uint64_t m = __ballot(ok[r]);
                     if (m == 0) continue;
                     uint32_t at = 0;
@@ -275,7 +279,7 @@ __global__ void __launch_bounds__(kBlock) k_bin_expand(const Inst* insts, InstSt
                             e[r];
                 }
                 __syncthreads();
-                if (sh.nout > kBinOutCap - kBlock * kBinUnroll) bin_out_flush(&sh, st, ctl, rp, rec0, lim);
+                if (sh.nout > sh.kOutCap - kBlock * EU) bin_out_flush(&sh, st, ctl, rp, rec0, lim);
             }
             __syncthreads();  // the batch tables are refilled next round
         }
