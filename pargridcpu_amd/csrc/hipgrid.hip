@@ -131,6 +131,8 @@ struct hg_graph {
     uint32_t bin_steps = 0;  // steps 1..bin_steps may bin
     uint32_t bin_rounds = 0; // removal rounds 0..bin_rounds-1 of such a step may bin
     uint32_t bin_eu = 4;     // entries per thread in flight in k_bin_expand (4 or 8)
+    uint32_t bin_xg = 4;     // k_bin_expand blocks per CU
+    uint32_t bin_sg = 2, bin_ag = 3;  // k_bin_scatter / k_bin_apply blocks per CU
     BinGeo bgeo{};
     uint32_t* bin_cbw = nullptr;
     uint32_t* bin_bh = nullptr;
@@ -191,11 +193,16 @@ struct Scratch {
 
 // Coarse bins of the binned apply: vertex ranges of about D / C_target
 // entries each (from the merged offsets: hub ranges are short), none longer
-// than kBinMaxF fine bins of vb = kBinDeltaWords / L vertices.  A graph whose
+// than kBinMaxF fine bins of vb = (the apply's LDS counter words) / L vertices.  A graph whose
 // table would exceed kBinMaxC bins keeps the atomic path everywhere.
 int bin_setup(hg_graph* g, Scratch& tmp, hgc::Staging& up) {
     const uint32_t L = g->L, n = g->n;
-    const uint32_t vb = std::min<uint32_t>(kBinDeltaWords / L, kBinMaxVb);
+    // blocks per CU of the scatter (2 or 3) and the apply (2 to 4): their LDS sizes
+    g->bin_sg = env_int("HG_BIN_SG", 2) == 3 ? 3u : 2u;
+    g->bin_ag = uint32_t(std::max(2, std::min(4, env_int("HG_BIN_AG", 3))));
+    const uint32_t dw = g->bin_ag == 4 ? kBinDeltaWords4 : g->bin_ag == 3 ? kBinDeltaWords3 : kBinDeltaWords2;
+    const uint32_t sw = g->bin_sg == 3 ? kBinStageWords3 : kBinStageWords2;
+    const uint32_t vb = std::min<uint32_t>(dw / L, kBinMaxVb);
     const uint64_t vlimit = uint64_t(vb) * kBinMaxF;
     const uint64_t ct = uint64_t(std::max(1, env_int("HG_BIN_C", 512)));
     const uint64_t E = std::max<uint64_t>(1, (g->D + ct - 1) / ct);
@@ -258,7 +265,7 @@ int bin_setup(hg_graph* g, Scratch& tmp, hgc::Staging& up) {
     bg.C = C;
     bg.cmax = C;
     bg.vbw = vb * L;
-    bg.stage = std::min<uint32_t>(uint32_t(kWave), kBinStageWords / C);
+    bg.stage = std::min<uint32_t>(uint32_t(kWave), sw / C);
     // binned when the frontier's entries reach HG_BIN_DENS / 1000 of the
     // alive vertices' degree words (default 100, i.e. 10 %: on the
     // 100M-vertex workload 496 ms per decomposition against 499 at 3 %)
@@ -270,6 +277,7 @@ int bin_setup(hg_graph* g, Scratch& tmp, hgc::Staging& up) {
     // the 100M-vertex workload one round is fastest (494 ms; 2: 498.5, 3: 500)
     g->bin_rounds = uint32_t(env_int("HG_BIN_ROUNDS", 1));
     g->bin_eu = env_int("HG_BIN_EU", 4) == 8 ? 8u : 4u;
+    g->bin_xg = uint32_t(std::min(8, env_int("HG_BIN_XG", 4)));
     g->bin_on = true;
     return HG_OK;
 }
@@ -612,22 +620,35 @@ int launch_step(hg_graph* g, uint32_t s) {
         for (uint32_t r = 0; r < g->bin_rounds; ++r) {  // removal rounds 0, 1, ... of the level
             double t[3] = {0, 0, 0};
             auto t0 = std::chrono::steady_clock::now();
+            const uint32_t gx = g->bin_xg * uint32_t(g->cus);
             if (g->bin_eu == 8)
-                hipLaunchKernelGGL(k_bin_expand<8>, dim3(g->G), dim3(kBlock), 0, g->stream, g->d_insts, g->d_states,
+                hipLaunchKernelGGL(k_bin_expand<8>, dim3(gx), dim3(kBlock), 0, g->stream, g->d_insts, g->d_states,
                                    g->d_ctl, L, s, r, g->bgeo);
             else
-                hipLaunchKernelGGL(k_bin_expand<4>, dim3(g->G), dim3(kBlock), 0, g->stream, g->d_insts, g->d_states,
+                hipLaunchKernelGGL(k_bin_expand<4>, dim3(gx), dim3(kBlock), 0, g->stream, g->d_insts, g->d_states,
                                    g->d_ctl, L, s, r, g->bgeo);
             if (blog && tick(0, t0)) return HG_ERR_HIP;
             t[0] = tk[0];
             t0 = std::chrono::steady_clock::now();
-            hipLaunchKernelGGL(k_bin_scatter, dim3(g2), dim3(kBlock), 0, g->stream, g->d_insts, g->d_states,
-                               g->d_ctl, L, s, r, g->bgeo);
+            if (g->bin_sg == 3)
+                hipLaunchKernelGGL(k_bin_scatter<kBinStageWords3>, dim3(3u * uint32_t(g->cus)), dim3(kBlock), 0,
+                                   g->stream, g->d_insts, g->d_states, g->d_ctl, L, s, r, g->bgeo);
+            else
+                hipLaunchKernelGGL(k_bin_scatter<kBinStageWords2>, dim3(g2), dim3(kBlock), 0, g->stream, g->d_insts,
+                                   g->d_states, g->d_ctl, L, s, r, g->bgeo);
             if (blog && tick(1, t0)) return HG_ERR_HIP;
             t[1] = tk[1];
             t0 = std::chrono::steady_clock::now();
-            hipLaunchKernelGGL(k_bin_apply, dim3(g2), dim3(kBlock), 0, g->stream, g->d_insts, g->d_states, g->d_ctl,
-                               L, s, r, g->bgeo);
+            const uint32_t ga = g->bin_ag * uint32_t(g->cus);
+            if (g->bin_ag == 4)
+                hipLaunchKernelGGL(k_bin_apply<kBinDeltaWords4>, dim3(ga), dim3(kBlock), 0, g->stream, g->d_insts,
+                                   g->d_states, g->d_ctl, L, s, r, g->bgeo);
+            else if (g->bin_ag == 3)
+                hipLaunchKernelGGL(k_bin_apply<kBinDeltaWords3>, dim3(ga), dim3(kBlock), 0, g->stream, g->d_insts,
+                                   g->d_states, g->d_ctl, L, s, r, g->bgeo);
+            else
+                hipLaunchKernelGGL(k_bin_apply<kBinDeltaWords2>, dim3(ga), dim3(kBlock), 0, g->stream, g->d_insts,
+                                   g->d_states, g->d_ctl, L, s, r, g->bgeo);
             if (blog && tick(2, t0)) return HG_ERR_HIP;
             t[2] = tk[2];
             for (int q = 0; q < 3; ++q) tsum[q] += t[q];

@@ -41,8 +41,10 @@ namespace {
 
 constexpr uint32_t kBinMaxC = 1024;         // coarse bins (LDS tables of the count and scatter kernels)
 constexpr uint32_t kBinMaxF = kBlock;       // fine bins per coarse bin (one per thread in the prefix sum)
-constexpr uint32_t kBinStageWords = 11264;  // scatter staging: C * S words
-constexpr uint32_t kBinDeltaWords = 16384;  // apply: LDS decrement counters of one fine bin
+// LDS sizes by blocks per CU: scatter staging (C * S words) and the apply's
+// decrement counters of one fine bin; 2 blocks per CU take the large ones.
+constexpr uint32_t kBinStageWords2 = 11264, kBinStageWords3 = 7168;
+constexpr uint32_t kBinDeltaWords2 = 16384, kBinDeltaWords3 = 9216, kBinDeltaWords4 = 6144;
 constexpr uint32_t kBinUnroll = 4;
 
 struct BinGeo {
@@ -109,7 +111,7 @@ __device__ __forceinline__ void bin_decide(const InstState* states, uint32_t I, 
     __syncthreads();
 }
 
-constexpr uint32_t kBinMaxG = 8192;  // word granules of the coarse-bin lookup table
+constexpr uint32_t kBinMaxG = 4096;  // word granules of the coarse-bin lookup table
 
 // Coarse bin of target word e: the largest c < C with cb[c] <= e, searched
 // among the bins that overlap e's granule (one bin in the sparse ranges).
@@ -291,6 +293,7 @@ __global__ void __launch_bounds__(kBlock) k_bin_expand(const Inst* insts, InstSt
     }
 }
 
+template <uint32_t SW>
 struct BinScatterShared {
     int32_t mode[64];
     unsigned long long rbase[64];
@@ -298,7 +301,7 @@ struct BinScatterShared {
     uint16_t gs[kBinMaxG + 1];
     uint32_t boff[kBinMaxC];   // region offset of each coarse bin within the instance's slice
     uint32_t fill[kBinMaxC];   // staging slots taken (may pass S while full; reset by the flush)
-    uint32_t stage[kBinStageWords];
+    uint32_t stage[SW];
     uint16_t full[kBinMaxC];
     uint32_t nfull;
     uint32_t tmp[kWavesPerBlock + 1];
@@ -310,7 +313,8 @@ struct BinScatterShared {
 // atomic on the bin's cursor, and empties those bins.  A run that would pass
 // the bin's count (cannot happen: the count pass counted these very records)
 // is dropped and flags the error.  All threads call.
-__device__ __forceinline__ void bin_flush(BinScatterShared* sh, uint32_t C, uint32_t S, uint32_t thresh,
+template <class SH>
+__device__ __forceinline__ void bin_flush(SH* sh, uint32_t C, uint32_t S, uint32_t thresh,
                                           uint32_t* bcur, const uint32_t* bcap, uint32_t* rec, uint64_t rb,
                                           Control* ctl) {
     for (uint32_t c = threadIdx.x; c < C; c += kBlock)
@@ -349,9 +353,10 @@ __device__ __forceinline__ void bin_flush(BinScatterShared* sh, uint32_t C, uint
 
 // Per binned instance: the records (expansion order) are streamed in equal
 // slices over the grid and moved into their coarse bins' regions of rec1.
+template <uint32_t SW>
 __global__ void __launch_bounds__(kBlock) k_bin_scatter(const Inst* insts, InstState* states, Control* ctl,
                                                         uint32_t I, uint32_t s, uint32_t round, BinGeo bg) {
-    __shared__ BinScatterShared sh;
+    __shared__ BinScatterShared<SW> sh;
     const uint32_t par = s & 1u;
     const uint32_t C = bg.C, S = bg.stage;
     if (threadIdx.x < I) {
@@ -433,11 +438,12 @@ __global__ void __launch_bounds__(kBlock) k_bin_scatter(const Inst* insts, InstS
 constexpr uint32_t kBinMaxVb = 4096;   // vertices per fine bin at most
 constexpr uint32_t kBinRemCap = 1024;  // removals listed in LDS before they are pushed
 
+template <uint32_t DW>
 struct BinApplyShared {
     int32_t mode[64];
     int32_t level[64];
     uint32_t first[65];                 // tasks of the instances before i
-    __attribute__((aligned(16))) uint32_t delta[kBinDeltaWords];  // records per degree word of the fine bin (zero between uses)
+    __attribute__((aligned(16))) uint32_t delta[DW];  // records per degree word of the fine bin (zero between uses)
     uint32_t rem[kBinRemCap];
     uint32_t nrem;
     uint32_t fstart[kBinMaxF + 1];
@@ -450,7 +456,8 @@ struct BinApplyShared {
 };
 
 // Pushes the listed removals as chunks for the cascade.  All threads call.
-__device__ __forceinline__ void bin_push_removed(const Inst& in, InstState* st, Control* ctl, BinApplyShared* sh,
+template <class SH>
+__device__ __forceinline__ void bin_push_removed(const Inst& in, InstState* st, Control* ctl, SH* sh,
                                                  uint32_t par, uint32_t rp) {
     const uint32_t nr = sh->nrem;
     for (uint32_t q0 = 0; q0 < nr; q0 += kBlock) {
@@ -472,7 +479,8 @@ __device__ __forceinline__ void bin_push_removed(const Inst& in, InstState* st, 
 // counts from its row (the task owns the row), clears them, stores the new
 // count of layers at or above thr and removes the vertex (index k) when
 // fewer than lmd are left; returns whether it did, with the vertex in *u.
-__device__ __forceinline__ bool bin_row_update(const Inst& in, BinApplyShared* sh, uint32_t uu, uint32_t fv0,
+template <class SH>
+__device__ __forceinline__ bool bin_row_update(const Inst& in, SH* sh, uint32_t uu, uint32_t fv0,
                                                uint32_t thr, int32_t k, bool quad, uint32_t* u) {
     const uint32_t L = in.L;
     uint32_t* dl = &sh->delta[uu * L];
@@ -512,7 +520,8 @@ __device__ __forceinline__ bool bin_row_update(const Inst& in, BinApplyShared* s
 // count subtracts them from its row (the task owns these rows), clears them,
 // and recounts its layers at or above the threshold; with fewer than lmd it
 // is removed.  The counters are zero again afterwards.  All threads call.
-__device__ __forceinline__ void bin_apply_fine(const Inst& in, InstState* st, Control* ctl, BinApplyShared* sh,
+template <class SH>
+__device__ __forceinline__ void bin_apply_fine(const Inst& in, InstState* st, Control* ctl, SH* sh,
                                                const uint32_t* recf, bool agent, uint32_t a, uint32_t b,
                                                uint32_t fw0, uint32_t nw, uint32_t par, uint32_t rp, int32_t k) {
     const uint32_t L = in.L, thr = uint32_t(k) + 1u;
@@ -579,8 +588,9 @@ __device__ __forceinline__ void bin_apply_fine(const Inst& in, InstState* st, Co
 // One (instance, coarse bin) task: the region's records grouped by fine bin
 // (unless the coarse bin is one fine bin), then each fine bin applied.  All
 // threads call.
+template <class SH>
 __device__ __forceinline__ void bin_apply_task(const Inst& in, InstState* st, Control* ctl, const BinGeo& bg,
-                                               BinApplyShared* sh, uint32_t i, uint32_t c, uint32_t par, uint32_t rp,
+                                               SH* sh, uint32_t i, uint32_t c, uint32_t par, uint32_t rp,
                                                int32_t k) {
     const uint64_t q = uint64_t(i) * bg.cmax + c;
     const uint32_t nrec = bg.bc[q];
@@ -650,15 +660,16 @@ __device__ __forceinline__ void bin_apply_task(const Inst& in, InstState* st, Co
 
 // Tasks (instance, coarse bin) of the binned instances, dealt round robin
 // over the grid.
+template <uint32_t DW>
 __global__ void __launch_bounds__(kBlock) k_bin_apply(const Inst* insts, InstState* states, Control* ctl, uint32_t I,
                                                       uint32_t s, uint32_t round, BinGeo bg) {
-    __shared__ BinApplyShared sh;
+    __shared__ BinApplyShared<DW> sh;
     const uint32_t par = s & 1u, rp = round & 1u;
     if (threadIdx.x < I) {
         sh.mode[threadIdx.x] = states[threadIdx.x].bflag[rp];
         sh.level[threadIdx.x] = states[threadIdx.x].dk[par];
     }
-    for (uint32_t t = threadIdx.x; t < kBinDeltaWords; t += kBlock) sh.delta[t] = 0;
+    for (uint32_t t = threadIdx.x; t < DW; t += kBlock) sh.delta[t] = 0;
     for (uint32_t t = threadIdx.x; t < kBinMaxVb / 32; t += kBlock) sh.claim[t] = 0;
     // the round's record count was read by the scatter; the next use of this
     // slot (two rounds on) counts from zero
