@@ -132,6 +132,7 @@ struct hg_graph {
     uint32_t bin_rounds = 0; // removal rounds 0..bin_rounds-1 of such a step may bin
     uint32_t bin_eu = 4;     // entries per thread in flight in k_bin_expand (4 or 8)
     uint32_t bin_xg = 4;     // k_bin_expand blocks per CU
+    bool bin_level0 = true;  // step 0 (level 0's pushed frontier) may bin too
     uint32_t bin_sg = 2, bin_ag = 3;  // k_bin_scatter / k_bin_apply blocks per CU
     BinGeo bgeo{};
     uint32_t* bin_cbw = nullptr;
@@ -277,6 +278,7 @@ int bin_setup(hg_graph* g, Scratch& tmp, hgc::Staging& up) {
     // the 100M-vertex workload one round is fastest (494 ms; 2: 498.5, 3: 500)
     g->bin_rounds = uint32_t(env_int("HG_BIN_ROUNDS", 1));
     g->bin_eu = env_int("HG_BIN_EU", 4) == 8 ? 8u : 4u;
+    g->bin_level0 = env_int("HG_BIN_L0", 1) == 1;
     g->bin_xg = uint32_t(std::min(8, env_int("HG_BIN_XG", 4)));
     g->bin_on = true;
     return HG_OK;
@@ -613,7 +615,7 @@ int launch_step(hg_graph* g, uint32_t s) {
         before.resize(L);
         HIP_TRY(hipMemcpy(before.data(), g->d_states, sizeof(InstState) * L, hipMemcpyDeviceToHost));
     }
-    if (s >= 1 && g->bin_on && s <= g->bin_steps) {
+    if (g->bin_on && s <= g->bin_steps && (s >= 1 || g->bin_level0)) {
         // dense first rounds: binned apply (the kernels return at once for
         // instances that do not bin this step)
         const uint32_t g2 = 2u * uint32_t(g->cus);
@@ -936,8 +938,12 @@ int decompose_impl(hg_graph* g) {
     }
     g->timing.levels = int64_t(s);
     g->timing.binned = g->h_ctl->binned;
-    g->timing.launches = int64_t(s) * 2 + 3 +
-                         (g->bin_on ? 3 * int64_t(g->bin_rounds) * int64_t(std::min<uint32_t>(s - 1, g->bin_steps)) : 0);
+    {
+        // steps s0..min(bin_steps, s-1) launched the binned kernels too
+        const int64_t s0 = g->bin_level0 ? 0 : 1, s1 = std::min<int64_t>(g->bin_steps, int64_t(s) - 1);
+        const int64_t nb = g->bin_on && s1 >= s0 ? s1 - s0 + 1 : 0;
+        g->timing.launches = int64_t(s) * 2 + 3 + 3 * int64_t(g->bin_rounds) * nb;
+    }
 
     g->h_count.assign(size_t(g->kcap) * L, 0);
     HIP_TRY(io.d2h(g->h_count.data(), g->d_ihist, sizeof(int64_t) * g->h_count.size()));

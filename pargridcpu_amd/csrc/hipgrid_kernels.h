@@ -651,6 +651,7 @@ k_init(const Inst* insts, InstState* states, Control* ctl, uint32_t I,
     for (uint32_t i = threadIdx.x; i < I; i += blockDim.x) {
         sh[i].keep = 0;
         sh[i].removed = 0;
+        sh[i].ent = 0;
     }
     step_decide(states, ctl, I, 0u, &ss);
     __syncthreads();
@@ -764,6 +765,14 @@ k_init(const Inst* insts, InstState* states, Control* ctl, uint32_t I,
                 if (alive)
                     in.list[lo + kb + __builtin_amdgcn_mbcnt_hi(uint32_t(km >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(km), 0u))] = v;
             }
+            {  // entries of the pushed vertices (the binned apply's density test)
+                const uint64_t pm = __ballot(push);
+                if (pm) {
+                    unsigned long long d = push ? (unsigned long long)(in.off[v + 1] - in.off[v]) : 0ull;
+                    d = wave_sum_u64(d);
+                    if (lane_id() == 0) atomicAdd(&sh[i].ent, d);
+                }
+            }
             block_defer_push(in, defer_list(in, 0u), &states[i].pfront[0][0], ctl, push, v, sh[i].tmp, &sh[i].base);
         }
     }
@@ -773,6 +782,7 @@ k_init(const Inst* insts, InstState* states, Control* ctl, uint32_t I,
         if (ss.phase[i] == kPhaseDone) continue;
         in.blk[2 * blockIdx.x] = sh[i].keep;
         if (sh[i].removed) atomicAdd(&states[i].alive[0], 0ull - (unsigned long long)sh[i].removed);
+        if (sh[i].ent) atomicAdd(&states[i].pfent[0][0], sh[i].ent);
     }
 }
 
