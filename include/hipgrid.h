@@ -179,6 +179,7 @@ int hg_last_timing(hg_graph* g, hg_timing* out);
 int hg_set_phase_timing(hg_graph* g, int32_t on);
 
 /* Graph facts for reporting. */
+/* device_bytes: the handle's device memory as allocated (cache blocks, rounded up). */
 int hg_graph_info(hg_graph* g, int64_t* n, int32_t* L, int64_t* directed_entries, int64_t* device_bytes);
 
 int hg_graph_destroy(hg_graph* g);
@@ -290,7 +291,12 @@ int hg_live_destroy(hg_live* q);
  * them and runs them as one batch (hg_batch_*, the throughput form); with
  * fewer waiting it takes one and runs it alone on the single-graph engine
  * (the latency form: the GPU is not saturated).  A request is admitted only
- * while the device bytes of all work in flight stay within `hbm_budget`.
+ * while the device bytes of all work in flight stay within `hbm_budget`,
+ * counted as the library's device-memory cache hands the blocks out (each
+ * rounded up to a power of two up to 512 MB, to 2 MB above); idle blocks the
+ * cache keeps between requests (up to HG_POOL_MB) are outside the budget:
+ * hg_server_create empties the cache, and a full device releases it before
+ * an allocation fails.
  * Requests that arrive while others run join the next batch a lane forms;
  * the lanes' work overlaps on the device.  Every answer equals that of
  * hg_graph_* on the request's graph alone.

@@ -444,8 +444,10 @@ int create_impl(const hg_layer_csr* layers, int32_t L, int64_t n, int32_t offset
 
     const size_t nL = size_t(n) * size_t(L);
     int64_t bytes = 0;
-    auto acct = [&](size_t b) { bytes += int64_t(b); };
-    acct(8 * n1 + 4 * size_t(n) + size_t(n));  // off, inv, z
+    auto acct = [&](size_t b) { bytes += int64_t(hgc::pool_block_bytes(b)); };  // as the cache hands blocks out
+    acct(8 * n1);  // off, inv, z
+    acct(4 * size_t(n));
+    acct(size_t(n));
     HIP_TRY(dalloc(&g->adj, g->D)); acct(4 * g->D);
     HIP_TRY(dalloc(&g->adjz, g->D + 16)); acct(g->D);  // the pull reads whole 16-byte words
     HIP_TRY(dalloc(&g->deg0, nL)); acct(4 * nL);

@@ -366,7 +366,7 @@ int create_impl(const hg_batch_graph* gs, int32_t count, void* stream, hg_batch*
     b->pull_nj = max_pulled <= 8 ? 8u : 16u;
 
     int64_t bytes = 0;
-    auto acct = [&](uint64_t x) { bytes += int64_t(x); };
+    auto acct = [&](uint64_t x) { bytes += int64_t(hgc::pool_block_bytes(x)); };  // as the cache hands blocks out
     HIP_TRY(dalloc(&b->off, O)); acct(4 * O);
     HIP_TRY(dalloc(&b->adj, E)); acct(4 * E);
     HIP_TRY(dalloc(&b->adjz, E + 16)); acct(E + 16);  // 16-byte reads past the last entry
@@ -718,7 +718,7 @@ int densest_impl(hg_batch* b, const int32_t* graphs, int32_t count, double beta,
     if (!b->histpool) {
         hipError_t e = dalloc(&b->histpool, std::max<uint64_t>(b->hist_words, 1));
         if (e != hipSuccess) return fail(HG_ERR_NOMEM, "batch densest histogram allocation failed");
-        b->device_bytes += int64_t(4 * b->hist_words);
+        b->device_bytes += int64_t(hgc::pool_block_bytes(4 * b->hist_words));
     }
     std::vector<uint32_t> sel;
     for (int32_t i = 0; i < count; ++i) {

@@ -180,6 +180,22 @@ hipError_t dalloc(T** p, size_t count) {
     return e;
 }
 
+// Bytes the cache hands out for a request of `bytes` (blocks of up to
+// kPoolMaxBlock rounded up to a power of two, at least 256; larger ones to
+// 2 MB granules; exact when the cache is off): what an allocation really
+// holds, which the byte accounting and the server's budget estimates count.
+inline uint64_t pool_block_bytes(uint64_t bytes) {
+    if (bytes == 0) bytes = 1;
+    if (dev_pool().cap == 0) return bytes;
+    if (bytes > kPoolMaxBlock) {
+        const uint64_t g = uint64_t(2) << 20;
+        return (bytes + g - 1) / g * g;
+    }
+    uint64_t c = 256;
+    while (c < bytes) c <<= 1;
+    return c;
+}
+
 // Binned apply of dense removal rounds (k_bin_*): used by graphs of at least
 // HG_BIN_MIN directed entries (default 2^26) and fewer than 2^32; HG_BIN=2
 // turns it off.  Its two record buffers hold bin_rcap(D) 4-byte records each.
@@ -191,7 +207,9 @@ inline uint64_t bin_rcap(uint64_t D) { return std::min<uint64_t>(3ull << 29, 2 *
 // Device bytes of the binned apply (record buffers and per-bin tables) for a
 // graph of D entries and L layers.
 inline uint64_t bin_bytes(uint64_t D, uint64_t L) {
-    return bin_enabled(D) ? 8 * bin_rcap(D) + 16 * 1025 * L + 4 * 1025 : 0;
+    if (!bin_enabled(D)) return 0;
+    return pool_block_bytes(8 * bin_rcap(D)) + 2 * pool_block_bytes(4 * 1024 * L) + pool_block_bytes(8 * 1024 * L) +
+           pool_block_bytes(4 * 1025) + pool_block_bytes(2 * 4097);
 }
 
 // Runs fn(begin, end) over [0, count) in contiguous ranges of at least

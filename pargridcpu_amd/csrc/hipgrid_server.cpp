@@ -103,21 +103,31 @@ double ms_between(Clock::time_point a, Clock::time_point b) {
 }
 
 // Device bytes a request holds while it runs (graph, per-lmd state,
-// set-up temporaries) on the single-graph engine and in a batch.
+// set-up temporaries) on the single-graph engine and in a batch, array by
+// array as the allocation cache hands the blocks out (rounded up, up to 2x
+// for blocks of at most 512 MB).  Idle blocks the cache keeps between
+// requests (up to HG_POOL_MB) are not counted: hg_server_create empties the
+// cache, and a full device releases it before an allocation fails.
 int64_t engine_bytes(const Request& r) {
-    const uint64_t n = uint64_t(r.n), L = uint64_t(r.L), D = r.D;
-    const uint64_t setup = 8 * L * (n + 1) + 4 * D + 64 * (n + 1) + 16 * n;  // CSR, sort and scan buffers
-    const uint64_t graph = 8 * (n + 1) + 5 * D + 4 * n * L + 5 * n + 4 * n * L + D / 8 + 8;
-    const uint64_t state = L * (4 * n * L + 8 * n + 8 * n + n / 8 + 16 * (D / 4096 + n + 1));
-    const uint64_t dens = 8 * L * (r.maxdeg + 2) * 4;
+    const uint64_t n = uint64_t(r.n), L = uint64_t(r.L), D = r.D, n1 = n + 1;
+    auto R = [](uint64_t b) { return hgc::pool_block_bytes(b); };
+    const uint64_t Dl = D / (L ? L : 1) + 1;  // a layer's entries (the upload is per layer)
+    const uint64_t setup = L * (R(8 * n1) + R(4 * Dl)) + 4 * R(8 * n1) + 2 * R(4 * n) + R(n) + 2 * R(16 * n1);
+    const uint64_t graph = R(8 * n1) + R(4 * n) + R(n) + R(4 * D) + R(D + 16) + 2 * R(4 * n * L) +
+                           R(4 * ((n + 31) / 32) * L) + R(8 * (D / 64 + n + 1));
+    const uint64_t gcap = n + 8192 * 64;  // per-block alive lists: n plus a run of slack per block
+    const uint64_t state = R(4 * n * L * L) + 2 * R(4 * n * L) + 2 * R(4 * gcap * L) + R(16 * (D / 256 + n + 1) * L);
+    const uint64_t dens = 4 * R(8 * L * (r.maxdeg + 2));
     return int64_t(graph + state + dens + setup + hgc::bin_bytes(D, L));
 }
 int64_t batch_bytes(const Request& r) {
-    const uint64_t n = uint64_t(r.n), L = uint64_t(r.L), D = r.D;
-    const uint64_t upload = 8 * L * (n + 1) + 4 * D + 8 * (n + 1) + 5 * n;
-    const uint64_t graph = 4 * (n + 1) + 5 * D + 5 * n + 4 * n * L + 4 * n * L;
-    const uint64_t state = L * (4 * n * L + 16 * n + 4 * (n / 32 + 1));
-    const uint64_t dens = r.kind == HG_REQ_DENSEST ? 4 * L * L * (r.maxdeg + 2) : 0;
+    const uint64_t n = uint64_t(r.n), L = uint64_t(r.L), D = r.D, n1 = n + 1;
+    auto R = [](uint64_t b) { return hgc::pool_block_bytes(b); };
+    const uint64_t Dl = D / (L ? L : 1) + 1;
+    const uint64_t upload = L * (R(8 * n1) + R(4 * Dl)) + R(8 * n1) + R(5 * n);
+    const uint64_t graph = R(4 * n1) + R(4 * D) + R(D + 16) + R(n) + R(4 * n * L) + R(4 * n) + R(4 * n * L);
+    const uint64_t state = R(4 * n * L * L) + 4 * R(4 * n * L) + R(4 * (n / 32 + 1) * L);
+    const uint64_t dens = r.kind == HG_REQ_DENSEST ? R(4 * L * L * (r.maxdeg + 2)) : 0;
     return int64_t(graph + state + dens + upload);
 }
 // Large requests always run on the engine; small ones may run either way.
