@@ -274,9 +274,9 @@ int bin_setup(hg_graph* g, Scratch& tmp, hgc::Staging& up) {
     bg.dens_num = dv && *dv ? uint32_t(std::max(0, std::atoi(dv))) : 100u;
     bg.dens_den = 1000;
     g->bin_steps = uint32_t(env_int("HG_BIN_STEPS", 64));
-    // removal rounds per level that may bin: later rounds are sparser, and on
-    // the 100M-vertex workload one round is fastest (494 ms; 2: 498.5, 3: 500)
-    g->bin_rounds = uint32_t(env_int("HG_BIN_ROUNDS", 1));
+    // removal rounds per level that may bin: later rounds are sparser; on the
+    // 100M-vertex workload 1: 469.6 ms, 2: 466.6, 3: 466.1
+    g->bin_rounds = uint32_t(env_int("HG_BIN_ROUNDS", 2));
     g->bin_eu = env_int("HG_BIN_EU", 4) == 8 ? 8u : 4u;
     g->bin_level0 = env_int("HG_BIN_L0", 1) == 1;
     g->bin_xg = uint32_t(std::min(8, env_int("HG_BIN_XG", 4)));
