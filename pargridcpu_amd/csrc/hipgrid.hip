@@ -279,6 +279,7 @@ int bin_setup(hg_graph* g, Scratch& tmp, hgc::Staging& up) {
     g->bin_rounds = uint32_t(env_int("HG_BIN_ROUNDS", 2));
     g->bin_eu = env_int("HG_BIN_EU", 4) == 8 ? 8u : 4u;
     g->bin_level0 = env_int("HG_BIN_L0", 1) == 1;
+    bg.cont = env_int("HG_BIN_CONT", 1) == 1 ? 1u : 0u;
     g->bin_xg = uint32_t(std::min(8, env_int("HG_BIN_XG", 4)));
     g->bin_on = true;
     return HG_OK;

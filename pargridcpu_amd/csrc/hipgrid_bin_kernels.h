@@ -62,6 +62,7 @@ struct BinGeo {
     uint32_t vbw;         // degree words per fine bin (vertices per fine bin * L)
     uint32_t stage;       // scatter staging slots per coarse bin (<= kWave)
     uint32_t dens_num, dens_den;  // ... and frontier entries * den >= num * alive vertices * L
+    uint32_t cont;                // continuation steps (deferred hub chunks) may bin too
 };
 
 struct BinDecide {
@@ -71,16 +72,17 @@ struct BinDecide {
 
 // Removal round r's input: the scan's frontier (r = 0) or the vertices round
 // r - 1 removed, which exists when round r - 1 binned.
-__device__ __forceinline__ bool bin_input(const InstState* st, uint32_t par, uint32_t r, unsigned long long* ent,
-                                          unsigned long long* chunks) {
-    if (r == 0) {
+__device__ __forceinline__ bool bin_input(const InstState* st, uint32_t par, uint32_t r, bool cont,
+                                          unsigned long long* ent, unsigned long long* chunks) {
+    const int32_t ph = st->dphase[par];
+    if (r == 0) {  // a scan step's frontier, or the hub chunks deferred to a continuation step
         *ent = st->pfent[par][0];
-        *chunks = st->pfront[par][0];
-        return st->dphase[par] == kPhaseScan;
+        *chunks = ph == kPhaseScan ? st->pfront[par][0] : st->pdefer[par][0];
+        return ph == kPhaseScan || (ph == kPhaseCont && cont);
     }
     *ent = st->pber[(r - 1) & 1][0];
     *chunks = st->pbr[(r - 1) & 1][0];
-    return st->dphase[par] == kPhaseScan && st->bflag[(r - 1) & 1] != 0;
+    return (ph == kPhaseScan || ph == kPhaseCont) && st->bflag[(r - 1) & 1] != 0;
 }
 
 // Which instances bin round r of this step, and their record slices
@@ -95,7 +97,7 @@ __device__ __forceinline__ void bin_decide(const InstState* states, uint32_t I, 
             const InstState* st = &states[i];
             int32_t mode = 0;
             unsigned long long rb = 0, ent = 0, chunks = 0;
-            if (bin_input(st, par, r, &ent, &chunks)) {
+            if (bin_input(st, par, r, bg.cont != 0, &ent, &chunks)) {
                 const unsigned long long al = st->alive[0];
                 if (ent >= bg.min_ent && ent * bg.dens_den >= (unsigned long long)bg.dens_num * al * L &&
                     base + ent <= bg.rcap) {
@@ -238,7 +240,7 @@ __global__ void __launch_bounds__(kBlock) k_bin_expand(const Inst* insts, InstSt
         const bool compact = (in.flags & kFlagCompact) != 0;
         const uint64_t rb = sh.d.rbase[i];
         unsigned long long lim, pc;  // the slice holds lim records: the input's entries
-        bin_input(st, par, round, &lim, &pc);
+        bin_input(st, par, round, bg.cont != 0, &lim, &pc);
         uint32_t* rec0 = bg.rec2 + rb;
         for (uint32_t t = threadIdx.x; t < bg.C; t += kBlock) sh.hist[t] = 0;
         if (threadIdx.x == 0) sh.nout = 0;

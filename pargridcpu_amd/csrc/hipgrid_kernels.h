@@ -195,6 +195,19 @@ __device__ __forceinline__ uint32_t wave_excl_scan(uint32_t x, uint32_t* total) 
     return s - x;
 }
 
+// Wave-wide maximum and sum (every lane gets the result).
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long x) {
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        const unsigned long long y = __shfl_xor(x, o, kWave);
+        x = y > x ? y : x;
+    }
+    return x;
+}
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long x) {
+    for (int o = kWave / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, kWave);
+    return x;
+}
+
 // Exclusive prefix sum over the block; s_tmp needs kWavesPerBlock + 1 words.
 // Must be called by every thread of the block.
 __device__ __forceinline__ uint32_t block_excl_scan(uint32_t x, uint32_t* total, uint32_t* s_tmp) {
@@ -226,7 +239,7 @@ __device__ __forceinline__ uint64_t* defer_list(const Inst& in, uint32_t parity)
 // Wave-aggregated append of the adjacency of the lanes' vertices (defer ==
 // true) to a deferred list, in chunks of kDefer entries.  All lanes call.
 __device__ __forceinline__ void defer_push(const Inst& in, uint64_t* list, unsigned long long* count, Control* ctl,
-                                           bool defer, uint32_t v) {
+                                           bool defer, uint32_t v, unsigned long long* s_ent = nullptr) {
     uint64_t beg = 0, dt = 0;
     uint32_t nch = 0;
     if (defer) {
@@ -235,6 +248,10 @@ __device__ __forceinline__ void defer_push(const Inst& in, uint64_t* list, unsig
         nch = uint32_t((dt + in.defer - 1) / in.defer);
     }
     if (__ballot(nch != 0) == 0) return;
+    if (s_ent) {  // the deferred entries, for the next step's binned apply
+        const unsigned long long d = wave_sum_u64(dt);
+        if (lane_id() == 0) atomicAdd(s_ent, d);
+    }
     uint32_t total;
     const uint32_t ex = wave_excl_scan(nch, &total);
     unsigned long long base = 0;
@@ -283,17 +300,6 @@ __device__ __forceinline__ void block_defer_push(const Inst& in, uint64_t* list,
 enum Prep : int { kPrepMaxDeg = 0, kPrepBad = 1, kPrepDefer = 2, kPrepHeavy = 3, kPrepMaxNz = 4, kPrepWords = 8 };
 enum PrepBad : unsigned long long { kBadRowPtr = 1, kBadCol = 2 };
 
-__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long x) {
-    for (int o = kWave / 2; o > 0; o >>= 1) {
-        const unsigned long long y = __shfl_xor(x, o, kWave);
-        x = y > x ? y : x;
-    }
-    return x;
-}
-__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long x) {
-    for (int o = kWave / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, kWave);
-    return x;
-}
 
 // 32-bit caller offsets widened to int64.
 __global__ void k_widen_offsets(const int32_t* src, uint64_t count, int64_t* dst) {
@@ -883,6 +889,7 @@ struct CascShared {
     uint32_t removed;
     uint32_t used;  // distinct keys in the table
     uint32_t heavy;  // this step's deferral threshold
+    unsigned long long dent;  // entries this block deferred for the current instance
 };
 
 // Places the vertices removed by this wavefront's lanes (rem) into the
@@ -925,7 +932,7 @@ __device__ __forceinline__ void emit_removed(const Inst& in, InstState* st, Cont
             placed = light;
         }
     }
-    defer_push(in, defer_list(in, next_parity), &st->pdefer[next_parity][0], ctl, rem && !placed, u);
+    defer_push(in, defer_list(in, next_parity), &st->pdefer[next_parity][0], ctl, rem && !placed, u, &sh->dent);
 }
 
 // Applies U decrements deg[e[r]] -= c[r] (where valid[r]) with the cascade
@@ -1130,6 +1137,7 @@ __global__ void __launch_bounds__(kBlock) k_cascade(const Inst* insts, InstState
             sh.head = 0;
             sh.tail = 0;
             sh.removed = 0;
+            sh.dent = 0;
         }
         __syncthreads();
         // frontier / deferred chunks
@@ -1163,6 +1171,7 @@ __global__ void __launch_bounds__(kBlock) k_cascade(const Inst* insts, InstState
         }
         if (threadIdx.x == 0) {
             if (sh.removed) atomicAdd(&st->alive[0], 0ull - (unsigned long long)sh.removed);
+            if (sh.dent) atomicAdd(&st->pfent[npar][0], sh.dent);
         }
         __syncthreads();
     }
