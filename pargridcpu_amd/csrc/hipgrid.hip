@@ -202,7 +202,6 @@ int bin_setup(hg_graph* g, Scratch& tmp, hgc::Staging& up) {
     g->bin_sg = env_int("HG_BIN_SG", 2) == 3 ? 3u : 2u;
     g->bin_ag = uint32_t(std::max(2, std::min(4, env_int("HG_BIN_AG", 3))));
     const uint32_t dw = g->bin_ag == 4 ? kBinDeltaWords4 : g->bin_ag == 3 ? kBinDeltaWords3 : kBinDeltaWords2;
-    const uint32_t sw = g->bin_sg == 3 ? kBinStageWords3 : kBinStageWords2;
     const uint32_t vb = std::min<uint32_t>(dw / L, kBinMaxVb);
     const uint64_t vlimit = uint64_t(vb) * kBinMaxF;
     const uint64_t ct = uint64_t(std::max(1, env_int("HG_BIN_C", 512)));
@@ -266,7 +265,6 @@ int bin_setup(hg_graph* g, Scratch& tmp, hgc::Staging& up) {
     bg.C = C;
     bg.cmax = C;
     bg.vbw = vb * L;
-    bg.stage = std::min<uint32_t>(uint32_t(kWave), sw / C);
     // binned when the frontier's entries reach HG_BIN_DENS / 1000 of the
     // alive vertices' degree words (default 100, i.e. 10 %: on the
     // 100M-vertex workload 496 ms per decomposition against 499 at 3 %)
@@ -636,10 +634,10 @@ int launch_step(hg_graph* g, uint32_t s) {
             t[0] = tk[0];
             t0 = std::chrono::steady_clock::now();
             if (g->bin_sg == 3)
-                hipLaunchKernelGGL(k_bin_scatter<kBinStageWords3>, dim3(3u * uint32_t(g->cus)), dim3(kBlock), 0,
+                hipLaunchKernelGGL(k_bin_scatter<kBinScatterU3>, dim3(3u * uint32_t(g->cus)), dim3(kBlock), 0,
                                    g->stream, g->d_insts, g->d_states, g->d_ctl, L, s, r, g->bgeo);
             else
-                hipLaunchKernelGGL(k_bin_scatter<kBinStageWords2>, dim3(g2), dim3(kBlock), 0, g->stream, g->d_insts,
+                hipLaunchKernelGGL(k_bin_scatter<kBinScatterU2>, dim3(g2), dim3(kBlock), 0, g->stream, g->d_insts,
                                    g->d_states, g->d_ctl, L, s, r, g->bgeo);
             if (blog && tick(1, t0)) return HG_ERR_HIP;
             t[1] = tk[1];

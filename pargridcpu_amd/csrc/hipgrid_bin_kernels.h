@@ -14,8 +14,9 @@
 //                  order) and counts the records per coarse bin (a coarse bin
 //                  is a range of vertices holding about D / C adjacency
 //                  entries, so the hub ranges are short);
-//   k_bin_scatter  streams the records into their coarse bins' regions (LDS
-//                  staging per bin, one global reservation per run);
+//   k_bin_scatter  streams the records into their coarse bins' regions (each
+//                  batch counting-sorted by bin in LDS, one global
+//                  reservation per bin and batch, coalesced runs);
 //   k_bin_apply    one workgroup per (instance, coarse bin): groups the
 //                  region's records by fine bin (vb vertices whose degree
 //                  words fit in LDS), counts them per degree word with LDS
@@ -41,9 +42,10 @@ namespace {
 
 constexpr uint32_t kBinMaxC = 1024;         // coarse bins (LDS tables of the count and scatter kernels)
 constexpr uint32_t kBinMaxF = kBlock;       // fine bins per coarse bin (one per thread in the prefix sum)
-// LDS sizes by blocks per CU: scatter staging (C * S words) and the apply's
-// decrement counters of one fine bin; 2 blocks per CU take the large ones.
-constexpr uint32_t kBinStageWords2 = 11264, kBinStageWords3 = 7168;
+// LDS sizes by blocks per CU: the scatter's records per thread and batch
+// (its sort buffer) and the apply's decrement counters of one fine bin; 2
+// blocks per CU take the large ones.
+constexpr uint32_t kBinScatterU2 = 22, kBinScatterU3 = 10;
 constexpr uint32_t kBinDeltaWords2 = 16384, kBinDeltaWords3 = 9216, kBinDeltaWords4 = 6144;
 constexpr uint32_t kBinUnroll = 4;
 
@@ -60,7 +62,6 @@ struct BinGeo {
     uint32_t ng, gsh;
     uint32_t C, cmax;
     uint32_t vbw;         // degree words per fine bin (vertices per fine bin * L)
-    uint32_t stage;       // scatter staging slots per coarse bin (<= kWave)
     uint32_t dens_num, dens_den;  // ... and frontier entries * den >= num * alive vertices * L
     uint32_t cont;                // continuation steps (deferred hub chunks) may bin too
 };
@@ -295,83 +296,49 @@ __global__ void __launch_bounds__(kBlock) k_bin_expand(const Inst* insts, InstSt
     }
 }
 
-template <uint32_t SW>
+template <uint32_t B>
 struct BinScatterShared {
     int32_t mode[64];
     unsigned long long rbase[64];
     uint32_t cb[kBinMaxC + 1];
     uint16_t gs[kBinMaxG + 1];
-    uint32_t boff[kBinMaxC];   // region offset of each coarse bin within the instance's slice
-    uint32_t fill[kBinMaxC];   // staging slots taken (may pass S while full; reset by the flush)
-    uint32_t stage[SW];
-    uint16_t full[kBinMaxC];
-    uint32_t nfull;
+    uint32_t boff[kBinMaxC];  // region offset of each coarse bin within the instance's slice
+    uint32_t hist[kBinMaxC];  // the batch's records per bin, then each bin's first slot in `sorted` (zero between batches)
+    uint32_t wofs[kBinMaxC];  // slice position of sorted slot j of bin c: wofs[c] + j (mod 2^32)
+    uint32_t sorted[B];       // the batch's records grouped by bin
     uint32_t tmp[kWavesPerBlock + 1];
-    unsigned long long nbatch, niter, nflush;
+    uint32_t bad;             // a reservation passed its bin's count (cannot happen)
+    unsigned long long nbatch;
 };
 
-// Writes out the staged records of every coarse bin holding at least
-// `thresh` of them (S: the full ones; 1: all), reserving each run with one
-// atomic on the bin's cursor, and empties those bins.  A run that would pass
-// the bin's count (cannot happen: the count pass counted these very records)
-// is dropped and flags the error.  All threads call.
-template <class SH>
-__device__ __forceinline__ void bin_flush(SH* sh, uint32_t C, uint32_t S, uint32_t thresh,
-                                          uint32_t* bcur, const uint32_t* bcap, uint32_t* rec, uint64_t rb,
-                                          Control* ctl) {
-    for (uint32_t c = threadIdx.x; c < C; c += kBlock)
-        if (sh->fill[c] >= thresh) sh->full[atomicAdd(&sh->nfull, 1u)] = uint16_t(c);
-    __syncthreads();
-    const uint32_t nf = sh->nfull;
-    const uint32_t lane = lane_id();
-    if (threadIdx.x == 0) sh->nflush += nf;
-    // a wave takes 64 listed bins at once: their reservations go out together
-    // (one round trip), then the wave writes the runs one after another
-    for (uint32_t q0 = (threadIdx.x / kWave) * kWave; q0 < nf; q0 += kBlock) {
-        const uint32_t q = q0 + lane;
-        uint32_t c = 0, cnt = 0, base = 0;
-        if (q < nf) {
-            c = sh->full[q];
-            const uint32_t f = sh->fill[c];
-            cnt = f < S ? f : S;
-            base = atomicAdd(&bcur[c], cnt);
-            if (uint64_t(base) + cnt > bcap[c]) {
-                atomicCAS(&ctl->error, 0, int32_t(kErrBin));
-                cnt = 0;
-            }
-        }
-        const uint32_t nj = nf - q0 < uint32_t(kWave) ? nf - q0 : uint32_t(kWave);
-        for (uint32_t j = 0; j < nj; ++j) {
-            const uint32_t cj = __shfl(c, int(j), kWave), kj = __shfl(cnt, int(j), kWave);
-            const uint32_t bj = __shfl(base, int(j), kWave);
-            if (lane < kj) rec[rb + sh->boff[cj] + bj + lane] = sh->stage[cj * S + lane];
-        }
-    }
-    __syncthreads();
-    for (uint32_t q = threadIdx.x; q < nf; q += kBlock) sh->fill[sh->full[q]] = 0;
-    if (threadIdx.x == 0) sh->nfull = 0;
-    __syncthreads();
-}
-
 // Per binned instance: the records (expansion order) are streamed in equal
-// slices over the grid and moved into their coarse bins' regions of rec1.
-template <uint32_t SW>
+// slices over the grid, B = U * kBlock at a time.  A batch is counting-sorted
+// by coarse bin in LDS (the rank of each record within its bin comes from the
+// histogram's atomic), each bin holding records reserves its run with one
+// atomic on the bin's cursor, and the block writes the sorted batch out:
+// consecutive threads write consecutive slots of a run.
+template <uint32_t U>
 __global__ void __launch_bounds__(kBlock) k_bin_scatter(const Inst* insts, InstState* states, Control* ctl,
                                                         uint32_t I, uint32_t s, uint32_t round, BinGeo bg) {
-    __shared__ BinScatterShared<SW> sh;
+    constexpr uint32_t B = U * kBlock;
+    static_assert(B <= 65536 && kBinMaxC <= 65536, "bin and rank are packed in 16 bits each");
+    __shared__ BinScatterShared<B> sh;
     const uint32_t par = s & 1u;
-    const uint32_t C = bg.C, S = bg.stage;
+    const uint32_t C = bg.C;
     if (threadIdx.x < I) {
         sh.mode[threadIdx.x] = states[threadIdx.x].bflag[round & 1u];
         sh.rbase[threadIdx.x] = states[threadIdx.x].rbase[par];
     }
     for (uint32_t t = threadIdx.x; t <= C; t += kBlock) sh.cb[t] = bg.cbw[t];
     for (uint32_t t = threadIdx.x; t <= bg.ng; t += kBlock) sh.gs[t] = bg.gsel[t];
+    for (uint32_t t = threadIdx.x; t < C; t += kBlock) sh.hist[t] = 0;
     if (threadIdx.x == 0) {
-        sh.nfull = 0;
-        sh.nbatch = sh.niter = sh.nflush = 0;
+        sh.bad = 0;
+        sh.nbatch = 0;
     }
     __syncthreads();
+    // bins c0 = 2 t and c0 + 1 belong to thread t in the per-batch scan (C <= 2 kBlock)
+    const uint32_t c0 = 2 * threadIdx.x, c1 = c0 + 1;
     for (uint32_t i = 0; i < I; ++i) {
         if (!sh.mode[i]) continue;
         const uint32_t* bh = bg.bh + uint64_t(i) * bg.cmax;
@@ -379,8 +346,8 @@ __global__ void __launch_bounds__(kBlock) k_bin_scatter(const Inst* insts, InstS
         const uint64_t rb = sh.rbase[i];
         // region offsets: exclusive prefix of the counts (block 0 publishes them)
         uint32_t carry = 0;
-        for (uint32_t c0 = 0; c0 < C; c0 += kBlock) {
-            const uint32_t c = c0 + threadIdx.x;
+        for (uint32_t q0 = 0; q0 < C; q0 += kBlock) {
+            const uint32_t c = q0 + threadIdx.x;
             const uint32_t x = c < C ? bh[c] : 0u;
             uint32_t tot;
             const uint32_t ex = block_excl_scan(x, &tot, sh.tmp);
@@ -390,51 +357,72 @@ __global__ void __launch_bounds__(kBlock) k_bin_scatter(const Inst* insts, InstS
             }
             carry += tot;
         }
-        for (uint32_t t = threadIdx.x; t < C; t += kBlock) sh.fill[t] = 0;
         __syncthreads();
         const uint64_t R = states[i].r0cnt[round & 1u][0];
         const uint64_t lo = R * blockIdx.x / gridDim.x, hi = R * (blockIdx.x + 1) / gridDim.x;
         const uint32_t* rec0 = bg.rec2 + rb;
-        constexpr uint32_t U = 2 * kBinUnroll;  // records per thread and batch
-        for (uint64_t t0 = lo; t0 < hi; t0 += kBlock * U) {
-            uint32_t e[U], cb[U];
-            bool pend[U];
+        uint32_t* rec1 = bg.rec1 + rb;
+        for (uint64_t t0 = lo; t0 < hi; t0 += B) {
+            const uint32_t nb = hi - t0 < B ? uint32_t(hi - t0) : B;
+            uint32_t e[U], cr[U];  // record; its bin (high 16 bits) and rank in the bin (low 16)
 #pragma unroll
             for (uint32_t r = 0; r < U; ++r) {
-                const uint64_t t = t0 + r * kBlock + threadIdx.x;
-                pend[r] = t < hi;
-                e[r] = pend[r] ? rec0[t] : 0u;
+                const uint32_t j = r * kBlock + threadIdx.x;
+                e[r] = j < nb ? rec0[t0 + j] : 0u;
             }
 #pragma unroll
-            for (uint32_t r = 0; r < U; ++r) cb[r] = pend[r] ? coarse_of(sh.cb, sh.gs, bg.gsh, e[r]) : 0u;
-            if (threadIdx.x == 0) sh.nbatch++;
-            // stage; bins that fill up are written out and the rest retried
-            for (;;) {
-                if (threadIdx.x == 0) sh.niter++;
-                bool left = false;
-#pragma unroll
-                for (uint32_t r = 0; r < U; ++r) {
-                    if (!pend[r]) continue;
-                    const uint32_t slot = atomicAdd(&sh.fill[cb[r]], 1u);
-                    if (slot < S) {
-                        sh.stage[cb[r] * S + slot] = e[r];
-                        pend[r] = false;
-                    } else {
-                        left = true;
-                    }
+            for (uint32_t r = 0; r < U; ++r) {
+                const uint32_t j = r * kBlock + threadIdx.x;
+                cr[r] = 0;
+                if (j < nb) {
+                    const uint32_t c = coarse_of(sh.cb, sh.gs, bg.gsh, e[r]);
+                    cr[r] = (c << 16) | atomicAdd(&sh.hist[c], 1u);
                 }
-                const int more = __syncthreads_or(left ? 1 : 0);
-                bin_flush(&sh, C, S, S, bcur, bh, bg.rec1, rb, ctl);
-                if (!more) break;
             }
+            __syncthreads();
+            {
+                // the batch's first slot per bin, and one reservation per bin
+                // that holds records (both of a thread's go out together)
+                const uint32_t x0 = c0 < C ? sh.hist[c0] : 0u, x1 = c1 < C ? sh.hist[c1] : 0u;
+                uint32_t tot;
+                const uint32_t ex = block_excl_scan(x0 + x1, &tot, sh.tmp);
+                uint32_t g0 = 0, g1 = 0;
+                if (x0) g0 = atomicAdd(&bcur[c0], x0);
+                if (x1) g1 = atomicAdd(&bcur[c1], x1);
+                if (x0) {
+                    if (uint64_t(g0) + x0 > bh[c0]) sh.bad = 1;
+                    sh.hist[c0] = ex;
+                    sh.wofs[c0] = sh.boff[c0] + g0 - ex;
+                }
+                if (x1) {
+                    if (uint64_t(g1) + x1 > bh[c1]) sh.bad = 1;
+                    sh.hist[c1] = ex + x0;
+                    sh.wofs[c1] = sh.boff[c1] + g1 - (ex + x0);
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (uint32_t r = 0; r < U; ++r)
+                if (r * kBlock + threadIdx.x < nb) sh.sorted[sh.hist[cr[r] >> 16] + (cr[r] & 0xFFFFu)] = e[r];
+            __syncthreads();
+            if (c0 < C) sh.hist[c0] = 0;
+            if (c1 < C) sh.hist[c1] = 0;
+            if (sh.bad) {
+                // a run would pass its bin's region (the expand counted these
+                // very records): nothing more is written, the error is raised
+                if (threadIdx.x == 0) atomicCAS(&ctl->error, 0, int32_t(kErrBin));
+                break;
+            }
+            for (uint32_t j = threadIdx.x; j < nb; j += kBlock) {
+                const uint32_t x = sh.sorted[j];
+                rec1[sh.wofs[coarse_of(sh.cb, sh.gs, bg.gsh, x)] + j] = x;
+            }
+            if (threadIdx.x == 0) sh.nbatch++;
+            __syncthreads();  // `sorted` and `wofs` are refilled by the next batch
         }
-        bin_flush(&sh, C, S, 1u, bcur, bh, bg.rec1, rb, ctl);
+        if (sh.bad) break;
     }
-    if (threadIdx.x == 0 && sh.nbatch) {
-        atomicAdd(&ctl->dbg[0], sh.nbatch);
-        atomicAdd(&ctl->dbg[1], sh.niter);
-        atomicAdd(&ctl->dbg[2], sh.nflush);
-    }
+    if (threadIdx.x == 0 && sh.nbatch) atomicAdd(&ctl->dbg[0], sh.nbatch);
 }
 
 constexpr uint32_t kBinMaxVb = 4096;   // vertices per fine bin at most
