@@ -48,6 +48,7 @@ constexpr uint32_t kBinMaxF = kBlock;       // fine bins per coarse bin (one per
 constexpr uint32_t kBinScatterU2 = 22, kBinScatterU3 = 10;
 constexpr uint32_t kBinDeltaWords2 = 16384, kBinDeltaWords3 = 9216, kBinDeltaWords4 = 6144;
 constexpr uint32_t kBinUnroll = 4;
+constexpr uint32_t kBinFineU = 8;  // the apply's records per thread and batch when grouping by fine bin
 
 struct BinGeo {
     const uint32_t* cbw;  // [C+1] coarse bin boundaries as word indices (vertex * L): 0 = cbw[0] < ... < cbw[C] = n*L
@@ -438,6 +439,8 @@ struct BinApplyShared {
     uint32_t nrem;
     uint32_t fstart[kBinMaxF + 1];
     uint32_t fcur[kBinMaxF];
+    uint32_t fhist[kBinMaxF];  // a batch's records per fine bin, then its first slot (zero between batches)
+    uint32_t fwo[kBinMaxF];    // region position of a batch's sorted slot j of fine bin f: fwo[f] + j (mod 2^32)
     uint32_t tmp[kWavesPerBlock + 1];
     uint32_t claim[kBinMaxVb / 32];  // vertices of the fine bin already updated (record-driven path; zero between uses)
     unsigned long long base;
@@ -621,17 +624,56 @@ __device__ __forceinline__ void bin_apply_task(const Inst& in, InstState* st, Co
         }
         __syncthreads();
         uint32_t* r2 = bg.rec2 + base;
-        for (uint32_t t0 = threadIdx.x; t0 < nrec; t0 += kBlock * kBinUnroll) {
-            uint32_t e[kBinUnroll];
+        // grouped by fine bin a batch at a time: counting-sorted in LDS (the
+        // decrement counters hold the batch; they are zeroed again before
+        // they count), then written out in runs by consecutive threads
+        constexpr uint32_t FW = sizeof(sh->delta) / sizeof(uint32_t) / kBlock;
+        constexpr uint32_t FU = FW < kBinFineU ? FW : kBinFineU;  // records per thread and batch
+        constexpr uint32_t FB = FU * kBlock;
+        static_assert(FU >= 1 && FB <= 65536 && kBinMaxF <= 65536, "fine bin and rank are packed in 16 bits each");
+        uint32_t* buf = sh->delta;
+        for (uint32_t f = threadIdx.x; f < nf; f += kBlock) sh->fhist[f] = 0;
+        __syncthreads();
+        for (uint32_t t0 = 0; t0 < nrec; t0 += FB) {
+            const uint32_t nb = nrec - t0 < FB ? nrec - t0 : FB;
+            uint32_t e[FU], fr[FU];  // record; its fine bin (high 16 bits) and rank in the bin (low 16)
 #pragma unroll
-            for (uint32_t r = 0; r < kBinUnroll; ++r) {
-                const uint32_t t = t0 + r * kBlock;
-                e[r] = t < nrec ? r1[t] : 0xFFFFFFFFu;
+            for (uint32_t r = 0; r < FU; ++r) {
+                const uint32_t j = r * kBlock + threadIdx.x;
+                e[r] = j < nb ? r1[t0 + j] : 0xFFFFFFFFu;
             }
 #pragma unroll
-            for (uint32_t r = 0; r < kBinUnroll; ++r)
-                if (e[r] >= w0 && e[r] < w1) r2[atomicAdd(&sh->fcur[(e[r] - w0) / vbw], 1u)] = e[r];
+            for (uint32_t r = 0; r < FU; ++r) {
+                fr[r] = 0xFFFFFFFFu;
+                if (e[r] >= w0 && e[r] < w1) {
+                    const uint32_t f = (e[r] - w0) / vbw;
+                    fr[r] = (f << 16) | atomicAdd(&sh->fhist[f], 1u);
+                }
+            }
+            __syncthreads();
+            uint32_t tot;
+            {
+                const uint32_t x = threadIdx.x < nf ? sh->fhist[threadIdx.x] : 0u;
+                const uint32_t ex = block_excl_scan(x, &tot, sh->tmp);
+                if (threadIdx.x < nf) {
+                    sh->fhist[threadIdx.x] = ex;
+                    sh->fwo[threadIdx.x] = sh->fcur[threadIdx.x] - ex;
+                    sh->fcur[threadIdx.x] += x;
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (uint32_t r = 0; r < FU; ++r)
+                if (fr[r] != 0xFFFFFFFFu) buf[sh->fhist[fr[r] >> 16] + (fr[r] & 0xFFFFu)] = e[r];
+            __syncthreads();
+            if (threadIdx.x < nf) sh->fhist[threadIdx.x] = 0;
+            for (uint32_t j = threadIdx.x; j < tot; j += kBlock) {
+                const uint32_t x = buf[j];
+                r2[sh->fwo[(x - w0) / vbw] + j] = x;
+            }
+            __syncthreads();  // `buf` and `fwo` are refilled by the next batch
         }
+        for (uint32_t t = threadIdx.x; t < (nrec < FB ? nrec : FB); t += kBlock) buf[t] = 0;
         sync_stores();  // rec2 is read back below by other waves of the block
         for (uint32_t f = 0; f < nf; ++f) {
             const uint32_t a = sh->fstart[f], b = sh->fstart[f + 1];
