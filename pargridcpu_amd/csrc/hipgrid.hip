@@ -57,6 +57,7 @@ struct hg_graph {
     int64_t* off = nullptr;
     uint32_t* adj = nullptr;
     uint8_t* adjz = nullptr;
+    uint8_t* adjh = nullptr;  // [L][D] core-index bounds per entry (or null)
     uint32_t* deg0 = nullptr;
     uint32_t* inv = nullptr;
     uint8_t* z = nullptr;
@@ -149,7 +150,7 @@ void release(hg_graph* g) {
     if (!g) return;
     (void)hipSetDevice(g->device);
     (void)hipStreamSynchronize(g->stream);  // nothing queued may still use the memory
-    void* ptrs[] = {g->off, g->adj, g->adjz, g->deg0, g->alive, g->inv, g->z, g->out, g->chunks, g->deg, g->cnt, g->idx, g->lists, g->rings, g->blk, g->pl,
+    void* ptrs[] = {g->off, g->adj, g->adjz, g->adjh, g->deg0, g->alive, g->inv, g->z, g->out, g->chunks, g->deg, g->cnt, g->idx, g->lists, g->rings, g->blk, g->pl,
                     g->d_ihist, g->d_insts, g->d_states, g->d_ctl, g->hist, g->d_density, g->d_sel, g->d_cell_len,
                     g->d_pow};
     for (void* p : ptrs)
@@ -501,6 +502,20 @@ int create_impl(const hg_layer_csr* layers, int32_t L, int64_t n, int32_t offset
             if (!same) return fail(HG_ERR_HIP, "upload verify: layer " + std::to_string(l) + " differs on the device");
         }
     }
+    const uint32_t lmagic = g->L == 1 ? 0xFFFFFFFFu : uint32_t((1ull << 32) / g->L);  // div_L needs magic < 2^32
+    if (hgc::hb_enabled(g->D, g->L)) {
+        // core-index bounds per entry, from the per-vertex ones (freed on return)
+        uint8_t* hv = nullptr;
+        HIP_TRY(tmp.alloc(&hv, nL));
+        HIP_TRY(dalloc(&g->adjh, size_t(L) * g->D));
+        g->device_bytes += int64_t(hgc::hb_bytes(g->D, g->L));
+        const uint32_t gv = uint32_t(std::min<uint64_t>((uint64_t(n) + kBlock - 1) / kBlock, uint64_t(g->cus) * 16));
+        hipLaunchKernelGGL(k_build_vbound, dim3(std::max(gv, 1u)), dim3(kBlock), 0, g->stream, g->deg0, g->n, g->L, hv);
+        const uint32_t ge = uint32_t(std::min<uint64_t>((g->D + kBlock - 1) / kBlock, uint64_t(g->cus) * 16));
+        hipLaunchKernelGGL(k_build_adjh, dim3(std::max(ge, 1u)), dim3(kBlock), 0, g->stream, g->adj, g->D, g->L,
+                           lmagic, hv, g->adjh);
+        HIP_TRY(hipGetLastError());
+    }
     if (hgc::bin_enabled(g->D)) {
         int rc = bin_setup(g, tmp, up);
         if (rc) return rc;
@@ -515,6 +530,7 @@ int create_impl(const hg_layer_csr* layers, int32_t L, int64_t n, int32_t offset
         in.off = g->off;
         in.adj = g->adj;
         in.adjz = g->adjz;
+        in.adjh = g->adjh ? g->adjh + size_t(i) * g->D : nullptr;
         in.alive = g->alive + aw * i;
         in.deg = g->deg + size_t(i) * nL;
         in.cnt = g->cnt + size_t(i) * n;
@@ -529,7 +545,7 @@ int create_impl(const hg_layer_csr* layers, int32_t L, int64_t n, int32_t offset
         in.L = g->L;
         in.lmd = i + 1;
         in.cap = g->cap;
-        in.lmagic = g->L == 1 ? 0xFFFFFFFFu : uint32_t((1ull << 32) / g->L);  // div_L needs magic < 2^32
+        in.lmagic = lmagic;
         in.flags = env_int("HG_AGG", 1) == 2 ? 0u : kFlagAggregate;  // HG_AGG=2 disables combining
         in.heavy = uint32_t(env_int("HG_HEAVY", int(kHeavy)));
         in.tail_heavy = uint32_t(env_int("HG_TAIL_HEAVY", int(in.heavy)));

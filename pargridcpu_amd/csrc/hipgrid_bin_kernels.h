@@ -240,6 +240,7 @@ __global__ void __launch_bounds__(kBlock) k_bin_expand(const Inst* insts, InstSt
         const Inst& in = insts[i];
         InstState* st = &states[i];
         const bool compact = (in.flags & kFlagCompact) != 0;
+        const EntryFilter ef = entry_filter(in, st->dk[par]);
         const uint64_t rb = sh.d.rbase[i];
         unsigned long long lim, pc;  // the slice holds lim records: the input's entries
         bin_input(st, par, round, bg.cont != 0, &lim, &pc);
@@ -264,7 +265,7 @@ __global__ void __launch_bounds__(kBlock) k_bin_expand(const Inst* insts, InstSt
                     if (ok[r]) {
                         const uint64_t p = batch_pos(&sh.bs, j);
                         e[r] = in.adj[p];
-                        ok[r] = compact || in.adjz[p] >= in.lmd;
+                        ok[r] = compact || ef.z[p] >= ef.t;
                     }
                 }
 #pragma unroll

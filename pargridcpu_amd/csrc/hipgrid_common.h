@@ -212,6 +212,15 @@ inline uint64_t bin_bytes(uint64_t D, uint64_t L) {
            pool_block_bytes(4 * 1025) + pool_block_bytes(2 * 4097);
 }
 
+// Core-index bounds per entry (adjh): for each lmd, one byte per entry holding
+// min(h, 255), h = the lmd-th largest layer degree of the entry's target (no
+// vertex can have a core index above it).  Kept for graphs of at most 16
+// layers whose L * D bytes stay within 64 GiB; HG_HBOUND=0 turns them off.
+inline bool hb_enabled(uint64_t D, uint64_t L) {
+    return env_int("HG_HBOUND", 1) == 1 && L >= 1 && L <= 16 && D > 0 && L * D <= (1ull << 36);
+}
+inline uint64_t hb_bytes(uint64_t D, uint64_t L) { return hb_enabled(D, L) ? pool_block_bytes(L * D) : 0; }
+
 // Runs fn(begin, end) over [0, count) in contiguous ranges of at least
 // min_per_thread items on up to 16 host threads (the caller's included).
 template <typename F>

@@ -80,6 +80,7 @@ struct Inst {
     const int64_t* off;   // [n+1] merged adjacency offsets
     const uint32_t* adj;  // [D] entries u*L + l
     const uint8_t* adjz;  // [D] z(u) of each entry's target (layers in which u has an edge)
+    const uint8_t* adjh;  // [D] or null: min(h, 255), h = the lmd-th largest layer degree of the target
     uint32_t* alive;      // [ceil(n/32)] liveness bitmap
     uint32_t* deg;        // [n*L] residual layer degrees
     int32_t* cnt;         // [n]
@@ -105,6 +106,20 @@ __device__ __forceinline__ int32_t* cnt_of(const Inst& in, uint32_t v) { return 
 __device__ __forceinline__ bool is_alive(const Inst& in, uint32_t v) { return (in.alive[v >> 5] >> (v & 31)) & 1u; }
 __device__ __forceinline__ void clear_alive(const Inst& in, uint32_t v) {
     atomicAnd(&in.alive[v >> 5], ~(1u << (v & 31)));
+}
+// Pre-filter of an entry before its target's liveness probe (a streaming
+// byte instead of a random bitmap word): the target can be alive at level k
+// only if f.z[p] >= f.t.  With the core-index bounds, a target whose bound h
+// is below k left at an earlier level, and h >= 1 holds exactly when the
+// target has edges in at least lmd layers (the z test); without them, z(u) >=
+// lmd.  A saturated bound (255) never filters.
+struct EntryFilter {
+    const uint8_t* z;
+    uint32_t t;
+};
+__device__ __forceinline__ EntryFilter entry_filter(const Inst& in, int32_t k) {
+    if (in.adjh) return {in.adjh, k < 1 ? 1u : k > 255 ? 255u : uint32_t(k)};
+    return {in.adjz, in.lmd};
 }
 constexpr uint32_t kFlagAggregate = 1;
 // The instance's rows are a compacted copy (k_cmp_*): every target was alive
@@ -418,6 +433,36 @@ __global__ void k_build_adj(const int64_t* const* row_ptr, const uint32_t* const
         }
     }
     if (__ballot(bad) && lane == 0 && prep) atomicOr(&prep[kPrepBad], (unsigned long long)kBadCol);
+}
+
+// Per-vertex core-index bounds: hv[v * L + j] = min(the (j+1)-th largest of
+// v's layer degrees, 255), L <= 16 (a vertex of the (k, j+1) core has j+1
+// layers of degree >= k, so its index for lmd = j+1 is at most that degree).
+__global__ void k_build_vbound(const uint32_t* deg0, uint32_t n, uint32_t L, uint8_t* hv) {
+    for (uint64_t v = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; v < n; v += uint64_t(gridDim.x) * blockDim.x) {
+        uint32_t d[16];
+#pragma unroll
+        for (uint32_t l = 0; l < 16; ++l) d[l] = l < L ? deg0[v * L + l] : 0u;
+#pragma unroll
+        for (uint32_t l = 0; l < 16; ++l) {
+            if (l < L) {
+                uint32_t r = 0;  // rank of layer l, descending, ties by layer
+#pragma unroll
+                for (uint32_t m = 0; m < 16; ++m) r += uint32_t(m < L && (d[m] > d[l] || (d[m] == d[l] && m < l)));
+                hv[v * L + r] = uint8_t(d[l] < 255u ? d[l] : 255u);
+            }
+        }
+    }
+}
+
+// adjh[j * D + p] = hv[u * L + j] for the target u of entry p and every lmd
+// j + 1: the bound streams beside the entry.
+__global__ void k_build_adjh(const uint32_t* adj, uint64_t D, uint32_t L, uint32_t lmagic, const uint8_t* hv,
+                             uint8_t* adjh) {
+    for (uint64_t p = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; p < D; p += uint64_t(gridDim.x) * blockDim.x) {
+        const uint8_t* h = hv + uint64_t(div_L(adj[p], L, lmagic)) * L;
+        for (uint32_t j = 0; j < L; ++j) adjh[uint64_t(j) * D + p] = h[j];
+    }
 }
 
 // out[i][w] = core index of instance i at internal id inv[w]: results in
@@ -1034,10 +1079,12 @@ __device__ __forceinline__ void cascade_expand(const Inst& in, InstState* st, Co
     sh->pre[threadIdx.x] = pre;
     sh->start[threadIdx.x] = start;
     __syncthreads();
-    const uint32_t L = in.L, magic = in.lmagic, lmd = in.lmd;
+    const uint32_t L = in.L, magic = in.lmagic;
     const uint32_t thr = uint32_t(k) + 1u;
     const uint32_t head = sh->head;
     const bool agg = (in.flags & kFlagAggregate) != 0;
+    const bool compact = (in.flags & kFlagCompact) != 0;
+    const EntryFilter ef = entry_filter(in, k);
     for (uint32_t j0 = 0; j0 < total; j0 += kBlock * kUnroll) {
         uint32_t e[kUnroll], u[kUnroll];
         bool live[kUnroll];
@@ -1055,8 +1102,7 @@ __device__ __forceinline__ void cascade_expand(const Inst& in, InstState* st, Co
                 }
                 const uint64_t p = sh->start[lo] + (j - sh->pre[lo]);
                 e[r] = in.adj[p];
-                // z(target) < lmd: never alive in this instance
-                live[r] = (in.flags & kFlagCompact) != 0 || in.adjz[p] >= lmd;
+                live[r] = compact || ef.z[p] >= ef.t;
             }
         }
 #pragma unroll

@@ -118,7 +118,9 @@ int64_t engine_bytes(const Request& r) {
     const uint64_t gcap = n + 8192 * 64;  // per-block alive lists: n plus a run of slack per block
     const uint64_t state = R(4 * n * L * L) + 2 * R(4 * n * L) + 2 * R(4 * gcap * L) + R(16 * (D / 256 + n + 1) * L);
     const uint64_t dens = 4 * R(8 * L * (r.maxdeg + 2));
-    return int64_t(graph + state + dens + setup + hgc::bin_bytes(D, L));
+    // set-up also holds the per-vertex bound bytes (n * L) while it builds adjh
+    return int64_t(graph + state + dens + setup + hgc::bin_bytes(D, L) + hgc::hb_bytes(D, L) +
+                   (hgc::hb_enabled(D, L) ? R(n * L) : 0));
 }
 int64_t batch_bytes(const Request& r) {
     const uint64_t n = uint64_t(r.n), L = uint64_t(r.L), D = r.D, n1 = n + 1;

@@ -376,6 +376,26 @@ def test_binned_rounds(monkeypatch, rounds):
             assert eng.timing().binned > 0
 
 
+@pytest.mark.parametrize("hbound,binned", [("1", False), ("0", False), ("1", True), ("0", True)])
+def test_core_bounds_dense(monkeypatch, hbound, binned):
+    """Per-entry core-index bounds (the lmd-th largest layer degree of the
+    target, saturated at 255) skip targets that left at an earlier level
+    without probing their liveness bit.  A dense graph whose core indices pass
+    255 for every lmd exercises the saturated bytes at levels above 255;
+    HG_HBOUND=0 keeps the z pre-filter only.  Cascade only and with every scan
+    step binned: exact against the oracle."""
+    monkeypatch.setenv("HG_HBOUND", hbound)
+    if binned:
+        force_binning(monkeypatch)
+    mg = MultilayerGraph.generate(small_spec(1500, 3, 500000, 54, gamma=3.0))
+    say(f"case dense bounds HG_HBOUND={hbound} binned={binned}")
+    ci, cl, kmax, t = run_engine(mg)
+    assert kmax.min() > 255
+    check_against_oracle(mg, ci, cl, kmax)
+    if binned:
+        assert t.binned > 0
+
+
 def test_binned_off_matches(monkeypatch):
     """HG_BIN=2 keeps every round on the atomic path (no binned rounds), with
     the same result."""
