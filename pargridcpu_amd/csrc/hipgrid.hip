@@ -279,6 +279,7 @@ int bin_setup(hg_graph* g, Scratch& tmp, hgc::Staging& up) {
     g->bin_eu = env_int("HG_BIN_EU", 4) == 8 ? 8u : 4u;
     g->bin_level0 = env_int("HG_BIN_L0", 1) == 1;
     bg.cont = env_int("HG_BIN_CONT", 1) == 1 ? 1u : 0u;
+    bg.claim_q = uint32_t(std::max(0, std::min(64, env_int("HG_BIN_CLAIM", 4))));
     g->bin_xg = uint32_t(std::min(8, env_int("HG_BIN_XG", 4)));
     g->bin_on = true;
     return HG_OK;
@@ -509,8 +510,10 @@ int create_impl(const hg_layer_csr* layers, int32_t L, int64_t n, int32_t offset
         HIP_TRY(tmp.alloc(&hv, nL));
         HIP_TRY(dalloc(&g->adjh, size_t(L) * g->D));
         g->device_bytes += int64_t(hgc::hb_bytes(g->D, g->L));
-        const uint32_t gv = uint32_t(std::min<uint64_t>((uint64_t(n) + kBlock - 1) / kBlock, uint64_t(g->cus) * 16));
-        hipLaunchKernelGGL(k_build_vbound, dim3(std::max(gv, 1u)), dim3(kBlock), 0, g->stream, g->deg0, g->n, g->L, hv);
+        const uint32_t gv = uint32_t(std::min<uint64_t>((uint64_t(n) * kWave + kBlock - 1) / kBlock,
+                                                        uint64_t(g->cus) * 16));
+        hipLaunchKernelGGL(k_build_vbound, dim3(std::max(gv, 1u)), dim3(kBlock), 0, g->stream, g->off, g->deg0,
+                           g->adjz, g->n, g->L, hv);
         const uint32_t ge = uint32_t(std::min<uint64_t>((g->D + kBlock - 1) / kBlock, uint64_t(g->cus) * 16));
         hipLaunchKernelGGL(k_build_adjh, dim3(std::max(ge, 1u)), dim3(kBlock), 0, g->stream, g->adj, g->D, g->L,
                            lmagic, hv, g->adjh);

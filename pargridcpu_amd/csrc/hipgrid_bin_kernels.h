@@ -65,6 +65,7 @@ struct BinGeo {
     uint32_t vbw;         // degree words per fine bin (vertices per fine bin * L)
     uint32_t dens_num, dens_den;  // ... and frontier entries * den >= num * alive vertices * L
     uint32_t cont;                // continuation steps (deferred hub chunks) may bin too
+    uint32_t claim_q;             // a fine bin of nv vertices is applied record by record below nv * claim_q / 4 records
 };
 
 struct BinDecide {
@@ -517,7 +518,8 @@ __device__ __forceinline__ bool bin_row_update(const Inst& in, SH* sh, uint32_t 
 template <class SH>
 __device__ __forceinline__ void bin_apply_fine(const Inst& in, InstState* st, Control* ctl, SH* sh,
                                                const uint32_t* recf, bool agent, uint32_t a, uint32_t b,
-                                               uint32_t fw0, uint32_t nw, uint32_t par, uint32_t rp, int32_t k) {
+                                               uint32_t fw0, uint32_t nw, uint32_t par, uint32_t rp, int32_t k,
+                                               uint32_t claim_q) {
     const uint32_t L = in.L, thr = uint32_t(k) + 1u;
     const uint32_t fv0 = fw0 / L, nv = nw / L;
     // every record lies in [fw0, fw0 + nw) (the passes before placed it
@@ -535,7 +537,7 @@ __device__ __forceinline__ void bin_apply_fine(const Inst& in, InstState* st, Co
     }
     __syncthreads();
     const bool quad = (L & 3u) == 0;
-    if (uint64_t(b - a) * 4 < nv) {
+    if (uint64_t(b - a) * 4 < uint64_t(nv) * claim_q) {
         // few records: each vertex is claimed by its first record (one LDS
         // bit) and updated once; work follows the records, not the vertices
         for (uint32_t t0 = a; t0 < b; t0 += kBlock) {
@@ -593,7 +595,7 @@ __device__ __forceinline__ void bin_apply_task(const Inst& in, InstState* st, Co
     const uint32_t vbw = bg.vbw;
     const uint32_t nf = (w1 - w0 + vbw - 1) / vbw;  // <= kBinMaxF (the host sizes coarse bins so)
     if (nrec > 0 && nf == 1) {
-        bin_apply_fine(in, st, ctl, sh, bg.rec1 + base, false, 0, nrec, w0, w1 - w0, par, rp, k);
+        bin_apply_fine(in, st, ctl, sh, bg.rec1 + base, false, 0, nrec, w0, w1 - w0, par, rp, k, bg.claim_q);
     } else if (nrec > 0) {
         for (uint32_t f = threadIdx.x; f <= nf; f += kBlock) sh->fstart[f] = 0;
         __syncthreads();
@@ -680,7 +682,8 @@ __device__ __forceinline__ void bin_apply_task(const Inst& in, InstState* st, Co
             const uint32_t a = sh->fstart[f], b = sh->fstart[f + 1];
             if (a == b) continue;  // no records: no row of this fine bin changes
             const uint32_t fw0 = w0 + f * vbw;
-            bin_apply_fine(in, st, ctl, sh, r2, true, a, b, fw0, (w1 - fw0) < vbw ? (w1 - fw0) : vbw, par, rp, k);
+            bin_apply_fine(in, st, ctl, sh, r2, true, a, b, fw0, (w1 - fw0) < vbw ? (w1 - fw0) : vbw, par, rp, k,
+                           bg.claim_q);
         }
     }
     bin_push_removed(in, st, ctl, sh, par, rp);

@@ -213,8 +213,9 @@ inline uint64_t bin_bytes(uint64_t D, uint64_t L) {
 }
 
 // Core-index bounds per entry (adjh): for each lmd, one byte per entry holding
-// min(h, 255), h = the lmd-th largest layer degree of the entry's target (no
-// vertex can have a core index above it).  Kept for graphs of at most 16
+// min(h, 255), h = the lmd-th largest of the target's layer degrees counting
+// only neighbours of z >= lmd (no vertex can have a core index above it for
+// that lmd).  Kept for graphs of at most 16
 // layers whose L * D bytes stay within 64 GiB; HG_HBOUND=0 turns them off.
 inline bool hb_enabled(uint64_t D, uint64_t L) {
     return env_int("HG_HBOUND", 1) == 1 && L >= 1 && L <= 16 && D > 0 && L * D <= (1ull << 36);

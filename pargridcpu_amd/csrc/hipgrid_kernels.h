@@ -80,7 +80,7 @@ struct Inst {
     const int64_t* off;   // [n+1] merged adjacency offsets
     const uint32_t* adj;  // [D] entries u*L + l
     const uint8_t* adjz;  // [D] z(u) of each entry's target (layers in which u has an edge)
-    const uint8_t* adjh;  // [D] or null: min(h, 255), h = the lmd-th largest layer degree of the target
+    const uint8_t* adjh;  // [D] or null: min(h, 255), h = the target's lmd-th largest layer degree among neighbours of z >= lmd
     uint32_t* alive;      // [ceil(n/32)] liveness bitmap
     uint32_t* deg;        // [n*L] residual layer degrees
     int32_t* cnt;         // [n]
@@ -435,22 +435,51 @@ __global__ void k_build_adj(const int64_t* const* row_ptr, const uint32_t* const
     if (__ballot(bad) && lane == 0 && prep) atomicOr(&prep[kPrepBad], (unsigned long long)kBadCol);
 }
 
-// Per-vertex core-index bounds: hv[v * L + j] = min(the (j+1)-th largest of
-// v's layer degrees, 255), L <= 16 (a vertex of the (k, j+1) core has j+1
-// layers of degree >= k, so its index for lmd = j+1 is at most that degree).
-__global__ void k_build_vbound(const uint32_t* deg0, uint32_t n, uint32_t L, uint8_t* hv) {
-    for (uint64_t v = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; v < n; v += uint64_t(gridDim.x) * blockDim.x) {
-        uint32_t d[16];
-#pragma unroll
-        for (uint32_t l = 0; l < 16; ++l) d[l] = l < L ? deg0[v * L + l] : 0u;
+// Per-vertex core-index bounds: hv[v * L + j] = min(h, 255), h = the
+// (j+1)-th largest over layers l of d_j(v, l), the entries of v in layer l
+// whose target has z > j -- the degrees instance lmd = j + 1 starts from
+// (no other neighbour is ever alive there).  A vertex of the (k, j+1) core
+// has j+1 layers of degree >= k, so its index is at most h; h = 0 exactly
+// for the vertices that instance removes at level 0 (or never counts).  One
+// wavefront per vertex: the counts by ballot, then lane j < L ranks the L
+// layer counts of instance j.  L <= 16.
+__global__ void k_build_vbound(const int64_t* off, const uint32_t* deg0, const uint8_t* adjz, uint32_t n, uint32_t L,
+                               uint8_t* hv) {
+    const uint32_t lane = lane_id();
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const uint32_t nwaves = gridDim.x * blockDim.x / kWave;
+    for (uint32_t v = wave; v < n; v += nwaves) {
+        uint64_t pos = uint64_t(off[v]);
+        uint32_t x[16];  // lane j: d_j(v, l) per layer l
 #pragma unroll
         for (uint32_t l = 0; l < 16; ++l) {
+            x[l] = 0;
             if (l < L) {
-                uint32_t r = 0;  // rank of layer l, descending, ties by layer
-#pragma unroll
-                for (uint32_t m = 0; m < 16; ++m) r += uint32_t(m < L && (d[m] > d[l] || (d[m] == d[l] && m < l)));
-                hv[v * L + r] = uint8_t(d[l] < 255u ? d[l] : 255u);
+                const uint32_t d = deg0[uint64_t(v) * L + l];
+                uint32_t mine = 0;
+                for (uint32_t t0 = 0; t0 < d; t0 += kWave) {
+                    const uint32_t z = t0 + lane < d ? uint32_t(adjz[pos + t0 + lane]) : 0u;
+                    for (uint32_t j = 0; j < L; ++j) {
+                        const uint32_t c = uint32_t(__popcll(__ballot(z > j)));
+                        if (lane == j) mine += c;
+                    }
+                }
+                x[l] = mine;
+                pos += d;
             }
+        }
+        if (lane < L) {
+            uint32_t h = 0;
+#pragma unroll
+            for (uint32_t l = 0; l < 16; ++l) {
+                if (l < L) {
+                    uint32_t r = 0;  // rank of layer l, descending, ties by layer
+#pragma unroll
+                    for (uint32_t m = 0; m < 16; ++m) r += uint32_t(m < L && (x[m] > x[l] || (x[m] == x[l] && m < l)));
+                    if (r == lane) h = x[l];
+                }
+            }
+            hv[uint64_t(v) * L + lane] = uint8_t(h < 255u ? h : 255u);
         }
     }
 }
