@@ -310,6 +310,39 @@ __device__ __forceinline__ void block_defer_push(const Inst& in, uint64_t* list,
     }
 }
 
+// The same append for U vertices per thread (defer[r] for v[r]): one block
+// scan and one atomic for all of them.  All threads call.
+template <uint32_t U>
+__device__ __forceinline__ void block_defer_push_n(const Inst& in, uint64_t* list, unsigned long long* count,
+                                                   Control* ctl, const bool* defer, const uint32_t* v,
+                                                   uint32_t* s_tmp, unsigned long long* s_base) {
+    uint32_t nch = 0;
+#pragma unroll
+    for (uint32_t r = 0; r < U; ++r)
+        if (defer[r]) nch += uint32_t((uint64_t(in.off[v[r] + 1] - in.off[v[r]]) + in.defer - 1) / in.defer);
+    uint32_t total;
+    const uint32_t ex = block_excl_scan(nch, &total, s_tmp);
+    if (total == 0) return;
+    if (threadIdx.x == 0) *s_base = atomicAdd(count, (unsigned long long)total);
+    __syncthreads();
+    const unsigned long long base = *s_base;
+    if (base + total > in.plcap) {  // cannot happen: plcap bounds all deferrals
+        if (threadIdx.x == 0) atomicCAS(&ctl->error, 0, int32_t(kErrDefer));
+        return;
+    }
+    uint32_t at = ex;
+#pragma unroll
+    for (uint32_t r = 0; r < U; ++r) {
+        if (!defer[r]) continue;
+        const uint64_t beg = uint64_t(in.off[v[r]]);
+        const uint64_t dt = uint64_t(in.off[v[r] + 1]) - beg;
+        for (uint64_t c0 = 0; c0 < dt; c0 += in.defer) {
+            const uint64_t rem = dt - c0;
+            list[base + at++] = make_item(beg + c0, rem < in.defer ? uint32_t(rem) : in.defer);
+        }
+    }
+}
+
 // ----------------------------------------------------------- graph set-up
 // Set-up words read back by the host (one allocation, zeroed first).
 enum Prep : int { kPrepMaxDeg = 0, kPrepBad = 1, kPrepDefer = 2, kPrepHeavy = 3, kPrepMaxNz = 4, kPrepWords = 8 };
@@ -574,30 +607,6 @@ struct ScanShared {
     unsigned long long base;
     unsigned long long ent;  // adjacency entries of the block's removals
 };
-
-// Survivors are appended to the block's alive list (in place: positions
-// never pass the chunk just read); removed vertices become chunks on the
-// frontier list of this step.  All threads call.
-__device__ __forceinline__ void scan_emit(const Inst& in, InstState* st, Control* ctl, ScanShared* sh,
-                                          uint32_t parity, bool keep, bool rem, uint32_t v, uint32_t* list) {
-    const uint64_t km = __ballot(keep);
-    if (km) {
-        uint32_t base = 0;
-        if (lane_id() == 0) base = atomicAdd(&sh->keep, uint32_t(__popcll(km)));
-        base = __shfl(base, 0, kWave);
-        if (keep) list[base + __builtin_amdgcn_mbcnt_hi(uint32_t(km >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(km), 0u))] = v;
-    }
-    const uint64_t rm = __ballot(rem);
-    if (rm) {
-        unsigned long long d = rem ? (unsigned long long)(in.off[v + 1] - in.off[v]) : 0ull;
-        d = wave_sum_u64(d);
-        if (lane_id() == 0) {
-            atomicAdd(&sh->removed, uint32_t(__popcll(rm)));
-            atomicAdd(&sh->ent, d);
-        }
-    }
-    block_defer_push(in, defer_list(in, parity), &st->pfront[parity][0], ctl, rem, v, sh->tmp, &sh->base);
-}
 
 // ------------------------------------------------------------- level 0
 // Level 0 by pull (L <= kPullMaxL).  With z(v) = number of layers in which v
@@ -885,8 +894,11 @@ __device__ __forceinline__ uint32_t row_count_ge(const uint32_t* d, uint32_t L, 
 
 // Level k >= 1 scan of every instance in phase Scan: walk the block's alive
 // list, drop vertices removed since (idx >= 0), remove those with fewer than
-// lmd layers of degree >= k+1, keep the rest with their cnt.  The list is
-// compacted in place (writes never pass the chunk being read).
+// lmd layers of degree >= k+1, keep the rest with their cnt.  Each thread
+// takes kScanU list entries per round, their loads issued together (the walk
+// is a chain of dependent loads: list, liveness word, row).  The list is
+// compacted in place (writes never pass the round being read).
+constexpr uint32_t kScanU = 4;
 __global__ void __launch_bounds__(kBlock) k_scan(const Inst* insts, InstState* states, Control* ctl, uint32_t I,
                                                  uint32_t s) {
     __shared__ ScanShared sh;
@@ -908,25 +920,49 @@ __global__ void __launch_bounds__(kBlock) k_scan(const Inst* insts, InstState* s
             sh.ent = 0;
         }
         __syncthreads();
-        for (uint32_t base = 0; base < len; base += kBlock) {
-            const uint32_t t = base + threadIdx.x;
-            uint32_t v = 0;
-            bool live = false;
-            if (t < len) {
-                v = list[t];
-                live = is_alive(in, v);
+        for (uint32_t base = 0; base < len; base += kBlock * kScanU) {
+            uint32_t v[kScanU], c[kScanU];
+            bool live[kScanU], rem[kScanU];
+#pragma unroll
+            for (uint32_t r = 0; r < kScanU; ++r) {
+                const uint32_t t = base + r * kBlock + threadIdx.x;
+                v[r] = t < len ? list[t] : 0u;
             }
-            uint32_t c = 0;
-            if (live) c = row_count_ge(rec_of(in, v), L, thr);
-            const bool rem = live && c < in.lmd;
-            const bool keep = live && !rem;
-            if (rem) {
-                *idx_of(in, v) = k;
-                clear_alive(in, v);
+#pragma unroll
+            for (uint32_t r = 0; r < kScanU; ++r) live[r] = base + r * kBlock + threadIdx.x < len && is_alive(in, v[r]);
+#pragma unroll
+            for (uint32_t r = 0; r < kScanU; ++r) c[r] = live[r] ? row_count_ge(rec_of(in, v[r]), L, thr) : 0u;
+            unsigned long long d = 0;
+#pragma unroll
+            for (uint32_t r = 0; r < kScanU; ++r) {
+                rem[r] = live[r] && c[r] < in.lmd;
+                if (rem[r]) {
+                    *idx_of(in, v[r]) = k;
+                    clear_alive(in, v[r]);
+                    d += (unsigned long long)(in.off[v[r] + 1] - in.off[v[r]]);
+                }
+                if (live[r]) *cnt_of(in, v[r]) = int32_t(c[r]);
             }
-            if (live) *cnt_of(in, v) = int32_t(c);
-            __syncthreads();  // every entry of this chunk is read before compaction writes
-            scan_emit(in, st, ctl, &sh, s & 1u, keep, rem, v, list);
+            __syncthreads();  // every entry of this round is read before compaction writes
+#pragma unroll
+            for (uint32_t r = 0; r < kScanU; ++r) {
+                const bool keep = live[r] && !rem[r];
+                const uint64_t km = __ballot(keep);
+                if (km) {
+                    uint32_t kb = 0;
+                    if (lane_id() == 0) kb = atomicAdd(&sh.keep, uint32_t(__popcll(km)));
+                    kb = __shfl(kb, 0, kWave);
+                    if (keep)
+                        list[kb + __builtin_amdgcn_mbcnt_hi(uint32_t(km >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(km), 0u))] =
+                            v[r];
+                }
+                const uint64_t rm = __ballot(rem[r]);
+                if (rm && lane_id() == 0) atomicAdd(&sh.removed, uint32_t(__popcll(rm)));
+            }
+            d = wave_sum_u64(d);
+            if (d && lane_id() == 0) atomicAdd(&sh.ent, d);
+            block_defer_push_n<kScanU>(in, defer_list(in, s & 1u), &st->pfront[s & 1u][0], ctl, rem, v, sh.tmp,
+                                       &sh.base);
         }
         __syncthreads();
         if (threadIdx.x == 0) {
