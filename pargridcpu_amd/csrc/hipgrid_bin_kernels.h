@@ -522,8 +522,40 @@ __device__ __forceinline__ void bin_apply_fine(const Inst& in, InstState* st, Co
                                                uint32_t claim_q) {
     const uint32_t L = in.L, thr = uint32_t(k) + 1u;
     const uint32_t fv0 = fw0 / L, nv = nw / L;
+    const bool quad = (L & 3u) == 0;
+    const bool few = uint64_t(b - a) * 4 < uint64_t(nv) * claim_q;
     // every record lies in [fw0, fw0 + nw) (the passes before placed it
     // there); one that did not would be skipped, never index LDS out of range
+    if (few && b - a <= kBlock * kBinUnroll) {
+        // few records, all held in registers from the count to the claims
+        // (no second load): each vertex is claimed by its first record (one
+        // LDS bit) and updated once
+        uint32_t e[kBinUnroll];
+#pragma unroll
+        for (uint32_t r = 0; r < kBinUnroll; ++r) {
+            const uint32_t t = a + r * kBlock + threadIdx.x;
+            e[r] = t < b ? (agent ? ld_agent(&recf[t]) : recf[t]) : 0xFFFFFFFFu;
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < kBinUnroll; ++r)
+            if (e[r] - fw0 < nw) atomicAdd(&sh->delta[e[r] - fw0], 1u);
+        __syncthreads();
+        for (uint32_t r = 0; r < kBinUnroll && a + r * kBlock < b; ++r) {
+            bool rem = false;
+            uint32_t u = 0;
+            if (e[r] - fw0 < nw) {
+                const uint32_t uu = div_L(e[r] - fw0, L, in.lmagic);
+                const uint32_t bit = 1u << (uu & 31u);
+                if ((atomicOr(&sh->claim[uu >> 5], bit) & bit) == 0) rem = bin_row_update(in, sh, uu, fv0, thr, k, quad, &u);
+            }
+            if (rem) sh->rem[atomicAdd(&sh->nrem, 1u)] = u;
+            __syncthreads();
+            if (sh->nrem > kBinRemCap - kBlock) bin_push_removed(in, st, ctl, sh, par, rp);
+        }
+        for (uint32_t w = threadIdx.x; w < (nv + 31) / 32; w += kBlock) sh->claim[w] = 0;
+        __syncthreads();
+        return;
+    }
     for (uint32_t t0 = a + threadIdx.x; t0 < b; t0 += kBlock * kBinUnroll) {
         uint32_t e[kBinUnroll];
 #pragma unroll
@@ -536,8 +568,7 @@ __device__ __forceinline__ void bin_apply_fine(const Inst& in, InstState* st, Co
             if (e[r] - fw0 < nw) atomicAdd(&sh->delta[e[r] - fw0], 1u);
     }
     __syncthreads();
-    const bool quad = (L & 3u) == 0;
-    if (uint64_t(b - a) * 4 < uint64_t(nv) * claim_q) {
+    if (few) {
         // few records: each vertex is claimed by its first record (one LDS
         // bit) and updated once; work follows the records, not the vertices
         for (uint32_t t0 = a; t0 < b; t0 += kBlock) {
