@@ -142,6 +142,9 @@ struct hg_graph {
     uint64_t* bin_bb = nullptr;
     uint32_t* bin_rec = nullptr;  // rec1 then rec2
     uint16_t* bin_gsel = nullptr;
+    uint32_t* bin_ffirst = nullptr;  // [C+1] first fine bin of each coarse bin
+    uint16_t* bin_fcoarse = nullptr; // [nfine] coarse bin of each fine bin
+    uint2* bin_fse = nullptr;        // [L][nfine] fine bins' record ranges
 };
 
 namespace {
@@ -165,7 +168,8 @@ void release(hg_graph* g) {
             if (c.off[b]) hgc::dfree(c.off[b]);
             if (c.adj[b]) hgc::dfree(c.adj[b]);
         }
-    void* bptrs[] = {g->bin_cbw, g->bin_bh, g->bin_bc, g->bin_bb, g->bin_rec, g->bin_gsel};
+    void* bptrs[] = {g->bin_cbw, g->bin_bh, g->bin_bc, g->bin_bb, g->bin_rec, g->bin_gsel, g->bin_ffirst,
+                     g->bin_fcoarse, g->bin_fse};
     for (void* p : bptrs)
         if (p) hgc::dfree(p);
     void* cptrs[] = {g->c_len, g->c_nch, g->c_item_off, g->c_items, g->c_temp, g->c_words};
@@ -239,6 +243,13 @@ int bin_setup(hg_graph* g, Scratch& tmp, hgc::Staging& up) {
         }
         gsel[ng] = uint16_t(C - 1);
     }
+    // fine bins: vb vertices each, numbered over all coarse bins
+    std::vector<uint32_t> ffirst(size_t(C) + 1, 0);
+    for (uint32_t c = 0; c < C; ++c) ffirst[c + 1] = ffirst[c] + (cbw[c + 1] - cbw[c] + vb * L - 1) / (vb * L);
+    const uint32_t nfine = ffirst[C];
+    std::vector<uint16_t> fcoarse(nfine);
+    for (uint32_t c = 0; c < C; ++c)
+        for (uint32_t f = ffirst[c]; f < ffirst[c + 1]; ++f) fcoarse[f] = uint16_t(c);
     const uint64_t rcap = hgc::bin_rcap(g->D);
     const size_t tab = size_t(L) * C;
     HIP_TRY(dalloc(&g->bin_cbw, cbw.size()));
@@ -247,6 +258,11 @@ int bin_setup(hg_graph* g, Scratch& tmp, hgc::Staging& up) {
     HIP_TRY(dalloc(&g->bin_bb, tab));
     HIP_TRY(dalloc(&g->bin_rec, size_t(2 * rcap)));
     HIP_TRY(dalloc(&g->bin_gsel, gsel.size()));
+    HIP_TRY(dalloc(&g->bin_ffirst, ffirst.size()));
+    HIP_TRY(dalloc(&g->bin_fcoarse, fcoarse.size()));
+    HIP_TRY(dalloc(&g->bin_fse, size_t(L) * nfine));
+    HIP_TRY(up.h2d(g->bin_ffirst, ffirst.data(), sizeof(uint32_t) * ffirst.size()));
+    HIP_TRY(up.h2d(g->bin_fcoarse, fcoarse.data(), sizeof(uint16_t) * fcoarse.size()));
     HIP_TRY(up.h2d(g->bin_cbw, cbw.data(), sizeof(uint32_t) * cbw.size()));
     HIP_TRY(up.h2d(g->bin_gsel, gsel.data(), sizeof(uint16_t) * gsel.size()));
     HIP_TRY(hipMemsetAsync(g->bin_bh, 0, sizeof(uint32_t) * tab, g->stream));
@@ -266,6 +282,10 @@ int bin_setup(hg_graph* g, Scratch& tmp, hgc::Staging& up) {
     bg.C = C;
     bg.cmax = C;
     bg.vbw = vb * L;
+    bg.ffirst = g->bin_ffirst;
+    bg.fcoarse = g->bin_fcoarse;
+    bg.fse = g->bin_fse;
+    bg.nfine = nfine;
     // binned when the frontier's entries reach HG_BIN_DENS / 1000 of the
     // alive vertices' degree words (default 100, i.e. 10 %: on the
     // 100M-vertex workload 496 ms per decomposition against 499 at 3 %)
@@ -662,14 +682,16 @@ int launch_step(hg_graph* g, uint32_t s) {
             t[1] = tk[1];
             t0 = std::chrono::steady_clock::now();
             const uint32_t ga = g->bin_ag * uint32_t(g->cus);
+            hipLaunchKernelGGL(k_bin_group<kBinFineU>, dim3(ga), dim3(kBlock), 0, g->stream, g->d_states, g->d_ctl, L,
+                               r, g->bgeo);
             if (g->bin_ag == 4)
-                hipLaunchKernelGGL(k_bin_apply<kBinDeltaWords4>, dim3(ga), dim3(kBlock), 0, g->stream, g->d_insts,
+                hipLaunchKernelGGL(k_bin_fine<kBinDeltaWords4>, dim3(ga), dim3(kBlock), 0, g->stream, g->d_insts,
                                    g->d_states, g->d_ctl, L, s, r, g->bgeo);
             else if (g->bin_ag == 3)
-                hipLaunchKernelGGL(k_bin_apply<kBinDeltaWords3>, dim3(ga), dim3(kBlock), 0, g->stream, g->d_insts,
+                hipLaunchKernelGGL(k_bin_fine<kBinDeltaWords3>, dim3(ga), dim3(kBlock), 0, g->stream, g->d_insts,
                                    g->d_states, g->d_ctl, L, s, r, g->bgeo);
             else
-                hipLaunchKernelGGL(k_bin_apply<kBinDeltaWords2>, dim3(ga), dim3(kBlock), 0, g->stream, g->d_insts,
+                hipLaunchKernelGGL(k_bin_fine<kBinDeltaWords2>, dim3(ga), dim3(kBlock), 0, g->stream, g->d_insts,
                                    g->d_states, g->d_ctl, L, s, r, g->bgeo);
             if (blog && tick(2, t0)) return HG_ERR_HIP;
             t[2] = tk[2];

@@ -17,11 +17,14 @@
 //   k_bin_scatter  streams the records into their coarse bins' regions (each
 //                  batch counting-sorted by bin in LDS, one global
 //                  reservation per bin and batch, coalesced runs);
-//   k_bin_apply    one workgroup per (instance, coarse bin): groups the
-//                  region's records by fine bin (vb vertices whose degree
-//                  words fit in LDS), counts them per degree word with LDS
-//                  atomics, applies each word's count to the row it owns and
-//                  updates cnt from the layers that fell below the level's
+//   k_bin_group    one workgroup per (instance, coarse bin): groups the
+//                  region's records by fine bin and publishes each fine
+//                  bin's record range;
+//   k_bin_fine     the fine bins (vb vertices whose degree words fit in
+//                  LDS) dealt interleaved over the grid: a workgroup counts
+//                  a fine bin's records per degree word with LDS atomics,
+//                  applies each word's count to the row it owns and updates
+//                  cnt from the layers that fell below the level's
 //                  threshold: a vertex left with fewer than lmd layers at or
 //                  above it is removed (index k).  Its work is proportional
 //                  to the records, not to the vertices of the range.
@@ -66,6 +69,10 @@ struct BinGeo {
     uint32_t dens_num, dens_den;  // ... and frontier entries * den >= num * alive vertices * L
     uint32_t cont;                // continuation steps (deferred hub chunks) may bin too
     uint32_t claim_q;             // a fine bin of nv vertices is applied record by record below nv * claim_q / 4 records
+    const uint32_t* ffirst;       // [C+1] first fine bin (index over all coarse bins) of each coarse bin
+    const uint16_t* fcoarse;      // [nfine] coarse bin of each fine bin
+    uint2* fse;                   // [I][nfine] each fine bin's records [x, y) within its coarse bin's region
+    uint32_t nfine;
 };
 
 struct BinDecide {
@@ -429,24 +436,29 @@ __global__ void __launch_bounds__(kBlock) k_bin_scatter(const Inst* insts, InstS
 }
 
 constexpr uint32_t kBinMaxVb = 4096;   // vertices per fine bin at most
-constexpr uint32_t kBinRemCap = 1024;  // removals listed in LDS before they are pushed
+constexpr uint32_t kBinFineBatch = 128;  // fine bins a block lists at a time
+// Removals listed in LDS before they are pushed: room for two sweep rounds
+// of 512 vertices at 9216 counter words (3 blocks per CU still fit in LDS),
+// one round for the other sizes.
+constexpr uint32_t kBinRemCap = 1024;
+template <uint32_t DW>
+constexpr uint32_t bin_rem_cap() { return DW == kBinDeltaWords3 ? 1536u : kBinRemCap; }
 
 template <uint32_t DW>
 struct BinApplyShared {
     int32_t mode[64];
     int32_t level[64];
-    uint32_t first[65];                 // tasks of the instances before i
     __attribute__((aligned(16))) uint32_t delta[DW];  // records per degree word of the fine bin (zero between uses)
-    uint32_t rem[kBinRemCap];
+    static constexpr uint32_t kRem = bin_rem_cap<DW>();
+    uint32_t rem[kRem];
     uint32_t nrem;
-    uint32_t fstart[kBinMaxF + 1];
-    uint32_t fcur[kBinMaxF];
-    uint32_t fhist[kBinMaxF];  // a batch's records per fine bin, then its first slot (zero between batches)
-    uint32_t fwo[kBinMaxF];    // region position of a batch's sorted slot j of fine bin f: fwo[f] + j (mod 2^32)
     uint32_t tmp[kWavesPerBlock + 1];
+    uint4 fdesc[kBinFineBatch];       // listed fine bins: records [x, y), first word z, words w
+    unsigned long long foff[kBinFineBatch];  // their records' buffer offset from rec1
+    uint32_t nlist;
     uint32_t claim[kBinMaxVb / 32];  // vertices of the fine bin already updated (record-driven path; zero between uses)
     unsigned long long base;
-    unsigned long long ent;  // entries of the removals pushed by the current task
+    unsigned long long ent;  // entries of the removals this block pushed for the current instance
     uint32_t removed;
 };
 
@@ -510,6 +522,45 @@ __device__ __forceinline__ bool bin_row_update(const Inst& in, SH* sh, uint32_t 
     return true;
 }
 
+// bin_row_update for L = 8 with the row already loaded (x0, x1): the sweep
+// issues the loads of two rows before either is used.
+template <class SH>
+__device__ __forceinline__ bool bin_row_update8(const Inst& in, SH* sh, uint32_t uu, uint32_t fv0, uint32_t thr,
+                                                int32_t k, uint4 x0, uint4 x1) {
+    uint32_t* dl = &sh->delta[uu * 8];
+    const uint32_t u = fv0 + uu;
+    const uint4 d0 = *reinterpret_cast<const uint4*>(dl), d1 = *reinterpret_cast<const uint4*>(dl + 4);
+    // d <= x: each record is a removed neighbour the word still counts
+    x0.x -= d0.x;
+    x0.y -= d0.y;
+    x0.z -= d0.z;
+    x0.w -= d0.w;
+    x1.x -= d1.x;
+    x1.y -= d1.y;
+    x1.z -= d1.z;
+    x1.w -= d1.w;
+    uint4* row = reinterpret_cast<uint4*>(rec_of(in, u));
+    row[0] = x0;
+    row[1] = x1;
+    *reinterpret_cast<uint4*>(dl) = make_uint4(0u, 0u, 0u, 0u);
+    *reinterpret_cast<uint4*>(dl + 4) = make_uint4(0u, 0u, 0u, 0u);
+    const uint32_t c = uint32_t(x0.x >= thr) + uint32_t(x0.y >= thr) + uint32_t(x0.z >= thr) + uint32_t(x0.w >= thr) +
+                       uint32_t(x1.x >= thr) + uint32_t(x1.y >= thr) + uint32_t(x1.z >= thr) + uint32_t(x1.w >= thr);
+    *cnt_of(in, u) = int32_t(c);
+    if (c >= in.lmd) return false;  // the vertex was alive (at least lmd layers) before these records
+    *idx_of(in, u) = k;
+    clear_alive(in, u);
+    return true;
+}
+
+// Any record counted for vertex uu of the fine bin (L = 8).
+template <class SH>
+__device__ __forceinline__ bool bin_touched8(const SH* sh, uint32_t uu) {
+    const uint4 x = *reinterpret_cast<const uint4*>(&sh->delta[uu * 8]);
+    const uint4 y = *reinterpret_cast<const uint4*>(&sh->delta[uu * 8 + 4]);
+    return (x.x | x.y | x.z | x.w | y.x | y.y | y.z | y.w) != 0;
+}
+
 // The records [a, b) of one fine bin (degree words [fw0, fw0 + nw)): counted
 // per word in LDS, then a sweep over the bin's vertices: a vertex with any
 // count subtracts them from its row (the task owns these rows), clears them,
@@ -550,7 +601,7 @@ __device__ __forceinline__ void bin_apply_fine(const Inst& in, InstState* st, Co
             }
             if (rem) sh->rem[atomicAdd(&sh->nrem, 1u)] = u;
             __syncthreads();
-            if (sh->nrem > kBinRemCap - kBlock) bin_push_removed(in, st, ctl, sh, par, rp);
+            if (sh->nrem > SH::kRem - kBlock) bin_push_removed(in, st, ctl, sh, par, rp);
         }
         for (uint32_t w = threadIdx.x; w < (nv + 31) / 32; w += kBlock) sh->claim[w] = 0;
         __syncthreads();
@@ -583,10 +634,36 @@ __device__ __forceinline__ void bin_apply_fine(const Inst& in, InstState* st, Co
             }
             if (rem) sh->rem[atomicAdd(&sh->nrem, 1u)] = u;
             __syncthreads();
-            if (sh->nrem > kBinRemCap - kBlock) bin_push_removed(in, st, ctl, sh, par, rp);
+            if (sh->nrem > SH::kRem - kBlock) bin_push_removed(in, st, ctl, sh, par, rp);
         }
         for (uint32_t w = threadIdx.x; w < (nv + 31) / 32; w += kBlock) sh->claim[w] = 0;
         __syncthreads();
+        return;
+    }
+    if (L == 8 && SH::kRem >= 3 * kBlock) {
+        // two vertices per thread and round, their rows' loads in flight together
+        for (uint32_t ub = 0; ub < nv; ub += 2 * kBlock) {
+            const uint32_t u0 = ub + threadIdx.x, u1 = u0 + kBlock;
+            const bool t0 = u0 < nv && bin_touched8(sh, u0);
+            const bool t1 = u1 < nv && bin_touched8(sh, u1);
+            uint4 a0 = make_uint4(0u, 0u, 0u, 0u), a1 = a0, b0 = a0, b1 = a0;
+            if (t0) {
+                const uint4* r = reinterpret_cast<const uint4*>(rec_of(in, fv0 + u0));
+                a0 = r[0];
+                a1 = r[1];
+            }
+            if (t1) {
+                const uint4* r = reinterpret_cast<const uint4*>(rec_of(in, fv0 + u1));
+                b0 = r[0];
+                b1 = r[1];
+            }
+            const bool r0 = t0 && bin_row_update8(in, sh, u0, fv0, thr, k, a0, a1);
+            const bool r1 = t1 && bin_row_update8(in, sh, u1, fv0, thr, k, b0, b1);
+            if (r0) sh->rem[atomicAdd(&sh->nrem, 1u)] = fv0 + u0;
+            if (r1) sh->rem[atomicAdd(&sh->nrem, 1u)] = fv0 + u1;
+            __syncthreads();
+            if (sh->nrem > SH::kRem - 2 * kBlock) bin_push_removed(in, st, ctl, sh, par, rp);
+        }
         return;
     }
     for (uint32_t ub = 0; ub < nv; ub += kBlock) {
@@ -608,139 +685,40 @@ __device__ __forceinline__ void bin_apply_fine(const Inst& in, InstState* st, Co
         }
         if (rem) sh->rem[atomicAdd(&sh->nrem, 1u)] = u;
         __syncthreads();
-        if (sh->nrem > kBinRemCap - kBlock) bin_push_removed(in, st, ctl, sh, par, rp);
+        if (sh->nrem > SH::kRem - kBlock) bin_push_removed(in, st, ctl, sh, par, rp);
     }
 }
 
-// One (instance, coarse bin) task: the region's records grouped by fine bin
-// (unless the coarse bin is one fine bin), then each fine bin applied.  All
-// threads call.
-template <class SH>
-__device__ __forceinline__ void bin_apply_task(const Inst& in, InstState* st, Control* ctl, const BinGeo& bg,
-                                               SH* sh, uint32_t i, uint32_t c, uint32_t par, uint32_t rp,
-                                               int32_t k) {
-    const uint64_t q = uint64_t(i) * bg.cmax + c;
-    const uint32_t nrec = bg.bc[q];
-    const uint64_t base = bg.bb[q];
-    const uint32_t w0 = bg.cbw[c], w1 = bg.cbw[c + 1];
-    const uint32_t vbw = bg.vbw;
-    const uint32_t nf = (w1 - w0 + vbw - 1) / vbw;  // <= kBinMaxF (the host sizes coarse bins so)
-    if (nrec > 0 && nf == 1) {
-        bin_apply_fine(in, st, ctl, sh, bg.rec1 + base, false, 0, nrec, w0, w1 - w0, par, rp, k, bg.claim_q);
-    } else if (nrec > 0) {
-        for (uint32_t f = threadIdx.x; f <= nf; f += kBlock) sh->fstart[f] = 0;
-        __syncthreads();
-        const uint32_t* r1 = bg.rec1 + base;
-        for (uint32_t t0 = threadIdx.x; t0 < nrec; t0 += kBlock * kBinUnroll) {
-            uint32_t e[kBinUnroll];
-#pragma unroll
-            for (uint32_t r = 0; r < kBinUnroll; ++r) {
-                const uint32_t t = t0 + r * kBlock;
-                e[r] = t < nrec ? r1[t] : 0xFFFFFFFFu;
-            }
-#pragma unroll
-            for (uint32_t r = 0; r < kBinUnroll; ++r) {
-                if (e[r] == 0xFFFFFFFFu) continue;
-                if (e[r] >= w0 && e[r] < w1) atomicAdd(&sh->fstart[(e[r] - w0) / vbw], 1u);
-                else atomicCAS(&ctl->error, 0, int32_t(kErrBin));  // cannot happen: the scatter binned it here
-            }
-        }
-        __syncthreads();
-        {
-            const uint32_t x = threadIdx.x < nf ? sh->fstart[threadIdx.x] : 0u;
-            uint32_t tot;
-            const uint32_t ex = block_excl_scan(x, &tot, sh->tmp);
-            if (threadIdx.x < nf) {
-                sh->fstart[threadIdx.x] = ex;
-                sh->fcur[threadIdx.x] = ex;
-            }
-            if (threadIdx.x == 0) sh->fstart[nf] = tot;
-        }
-        __syncthreads();
-        uint32_t* r2 = bg.rec2 + base;
-        // grouped by fine bin a batch at a time: counting-sorted in LDS (the
-        // decrement counters hold the batch; they are zeroed again before
-        // they count), then written out in runs by consecutive threads
-        constexpr uint32_t FW = sizeof(sh->delta) / sizeof(uint32_t) / kBlock;
-        constexpr uint32_t FU = FW < kBinFineU ? FW : kBinFineU;  // records per thread and batch
-        constexpr uint32_t FB = FU * kBlock;
-        static_assert(FU >= 1 && FB <= 65536 && kBinMaxF <= 65536, "fine bin and rank are packed in 16 bits each");
-        uint32_t* buf = sh->delta;
-        for (uint32_t f = threadIdx.x; f < nf; f += kBlock) sh->fhist[f] = 0;
-        __syncthreads();
-        for (uint32_t t0 = 0; t0 < nrec; t0 += FB) {
-            const uint32_t nb = nrec - t0 < FB ? nrec - t0 : FB;
-            uint32_t e[FU], fr[FU];  // record; its fine bin (high 16 bits) and rank in the bin (low 16)
-#pragma unroll
-            for (uint32_t r = 0; r < FU; ++r) {
-                const uint32_t j = r * kBlock + threadIdx.x;
-                e[r] = j < nb ? r1[t0 + j] : 0xFFFFFFFFu;
-            }
-#pragma unroll
-            for (uint32_t r = 0; r < FU; ++r) {
-                fr[r] = 0xFFFFFFFFu;
-                if (e[r] >= w0 && e[r] < w1) {
-                    const uint32_t f = (e[r] - w0) / vbw;
-                    fr[r] = (f << 16) | atomicAdd(&sh->fhist[f], 1u);
-                }
-            }
-            __syncthreads();
-            uint32_t tot;
-            {
-                const uint32_t x = threadIdx.x < nf ? sh->fhist[threadIdx.x] : 0u;
-                const uint32_t ex = block_excl_scan(x, &tot, sh->tmp);
-                if (threadIdx.x < nf) {
-                    sh->fhist[threadIdx.x] = ex;
-                    sh->fwo[threadIdx.x] = sh->fcur[threadIdx.x] - ex;
-                    sh->fcur[threadIdx.x] += x;
-                }
-            }
-            __syncthreads();
-#pragma unroll
-            for (uint32_t r = 0; r < FU; ++r)
-                if (fr[r] != 0xFFFFFFFFu) buf[sh->fhist[fr[r] >> 16] + (fr[r] & 0xFFFFu)] = e[r];
-            __syncthreads();
-            if (threadIdx.x < nf) sh->fhist[threadIdx.x] = 0;
-            for (uint32_t j = threadIdx.x; j < tot; j += kBlock) {
-                const uint32_t x = buf[j];
-                r2[sh->fwo[(x - w0) / vbw] + j] = x;
-            }
-            __syncthreads();  // `buf` and `fwo` are refilled by the next batch
-        }
-        for (uint32_t t = threadIdx.x; t < (nrec < FB ? nrec : FB); t += kBlock) buf[t] = 0;
-        sync_stores();  // rec2 is read back below by other waves of the block
-        for (uint32_t f = 0; f < nf; ++f) {
-            const uint32_t a = sh->fstart[f], b = sh->fstart[f + 1];
-            if (a == b) continue;  // no records: no row of this fine bin changes
-            const uint32_t fw0 = w0 + f * vbw;
-            bin_apply_fine(in, st, ctl, sh, r2, true, a, b, fw0, (w1 - fw0) < vbw ? (w1 - fw0) : vbw, par, rp, k,
-                           bg.claim_q);
-        }
-    }
-    bin_push_removed(in, st, ctl, sh, par, rp);
-    if (threadIdx.x == 0) {
-        bg.bh[q] = 0;
-        bg.bc[q] = 0;
-        if (nrec) atomicAdd(&st->brec[0], (unsigned long long)nrec);
-    }
-}
+// Grouping of the coarse bins' records by fine bin: a batch of the region's
+// records counting-sorted in LDS, written out in runs by consecutive threads.
+template <uint32_t FU>
+struct BinGroupShared {
+    int32_t mode[64];
+    uint32_t first[65];                 // tasks of the instances before i
+    uint32_t fstart[kBinMaxF + 1];
+    uint32_t fcur[kBinMaxF];
+    uint32_t fhist[kBinMaxF];  // a batch's records per fine bin, then its first slot (zero between batches)
+    uint32_t fwo[kBinMaxF];    // region position of a batch's sorted slot j of fine bin f: fwo[f] + j (mod 2^32)
+    uint32_t buf[FU * kBlock];
+    uint32_t tmp[kWavesPerBlock + 1];
+};
 
 // Tasks (instance, coarse bin) of the binned instances, dealt round robin
-// over the grid.
-template <uint32_t DW>
-__global__ void __launch_bounds__(kBlock) k_bin_apply(const Inst* insts, InstState* states, Control* ctl, uint32_t I,
-                                                      uint32_t s, uint32_t round, BinGeo bg) {
-    __shared__ BinApplyShared<DW> sh;
-    const uint32_t par = s & 1u, rp = round & 1u;
-    if (threadIdx.x < I) {
-        sh.mode[threadIdx.x] = states[threadIdx.x].bflag[rp];
-        sh.level[threadIdx.x] = states[threadIdx.x].dk[par];
-    }
-    for (uint32_t t = threadIdx.x; t < DW; t += kBlock) sh.delta[t] = 0;
-    for (uint32_t t = threadIdx.x; t < kBinMaxVb / 32; t += kBlock) sh.claim[t] = 0;
+// over the grid: a coarse bin of several fine bins has its region's records
+// grouped by fine bin into rec2 (same region); every fine bin's record range
+// is published in fse (a coarse bin of one fine bin keeps its records in
+// rec1).  The bin's counts are cleared for the next round.
+template <uint32_t FU>
+__global__ void __launch_bounds__(kBlock) k_bin_group(InstState* states, Control* ctl, uint32_t I, uint32_t round,
+                                                      BinGeo bg) {
+    constexpr uint32_t FB = FU * kBlock;
+    static_assert(FB <= 65536 && kBinMaxF <= 65536, "fine bin and rank are packed in 16 bits each");
+    __shared__ BinGroupShared<FU> sh;
+    const uint32_t rp = round & 1u;
+    if (threadIdx.x < I) sh.mode[threadIdx.x] = states[threadIdx.x].bflag[rp];
     // the round's record count was read by the scatter; the next use of this
     // slot (two rounds on) counts from zero
-    if (blockIdx.x == 0 && threadIdx.x < I && sh.mode[threadIdx.x]) states[threadIdx.x].r0cnt[rp][0] = 0;
+    if (blockIdx.x == 0 && threadIdx.x < I && states[threadIdx.x].bflag[rp]) states[threadIdx.x].r0cnt[rp][0] = 0;
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t t = 0;
@@ -749,20 +727,162 @@ __global__ void __launch_bounds__(kBlock) k_bin_apply(const Inst* insts, InstSta
             if (sh.mode[i]) t += bg.C;
         }
         sh.first[I] = t;
+    }
+    __syncthreads();
+    const uint32_t ntasks = sh.first[I];
+    const uint32_t vbw = bg.vbw;
+    for (uint32_t t = blockIdx.x; t < ntasks; t += gridDim.x) {
+        uint32_t i = 0;
+        while (!(sh.mode[i] && t < sh.first[i] + bg.C)) ++i;  // t < ntasks: some binned instance holds it
+        const uint32_t c = t - sh.first[i];
+        const uint64_t q = uint64_t(i) * bg.cmax + c;
+        const uint32_t nrec = bg.bc[q];
+        const uint64_t base = bg.bb[q];
+        const uint32_t w0 = bg.cbw[c], w1 = bg.cbw[c + 1];
+        const uint32_t nf = (w1 - w0 + vbw - 1) / vbw;  // <= kBinMaxF (the host sizes coarse bins so)
+        uint2* fse = bg.fse + uint64_t(i) * bg.nfine + bg.ffirst[c];
+        if (nf == 1 || nrec == 0) {
+            for (uint32_t f = threadIdx.x; f < nf; f += kBlock) fse[f] = make_uint2(0u, nf == 1 ? nrec : 0u);
+        } else {
+            for (uint32_t f = threadIdx.x; f <= nf; f += kBlock) sh.fstart[f] = 0;
+            for (uint32_t f = threadIdx.x; f < nf; f += kBlock) sh.fhist[f] = 0;
+            __syncthreads();
+            const uint32_t* r1 = bg.rec1 + base;
+            for (uint32_t t0 = threadIdx.x; t0 < nrec; t0 += kBlock * kBinUnroll) {
+                uint32_t e[kBinUnroll];
+#pragma unroll
+                for (uint32_t r = 0; r < kBinUnroll; ++r) {
+                    const uint32_t j = t0 + r * kBlock;
+                    e[r] = j < nrec ? r1[j] : 0xFFFFFFFFu;
+                }
+#pragma unroll
+                for (uint32_t r = 0; r < kBinUnroll; ++r) {
+                    if (e[r] == 0xFFFFFFFFu) continue;
+                    if (e[r] >= w0 && e[r] < w1) atomicAdd(&sh.fstart[(e[r] - w0) / vbw], 1u);
+                    else atomicCAS(&ctl->error, 0, int32_t(kErrBin));  // cannot happen: the scatter binned it here
+                }
+            }
+            __syncthreads();
+            {
+                const uint32_t x = threadIdx.x < nf ? sh.fstart[threadIdx.x] : 0u;
+                uint32_t tot;
+                const uint32_t ex = block_excl_scan(x, &tot, sh.tmp);
+                if (threadIdx.x < nf) {
+                    sh.fstart[threadIdx.x] = ex;
+                    sh.fcur[threadIdx.x] = ex;
+                }
+                if (threadIdx.x == 0) sh.fstart[nf] = tot;
+            }
+            __syncthreads();
+            uint32_t* r2 = bg.rec2 + base;
+            for (uint32_t t0 = 0; t0 < nrec; t0 += FB) {
+                const uint32_t nb = nrec - t0 < FB ? nrec - t0 : FB;
+                uint32_t e[FU], fr[FU];  // record; its fine bin (high 16 bits) and rank in the bin (low 16)
+#pragma unroll
+                for (uint32_t r = 0; r < FU; ++r) {
+                    const uint32_t j = r * kBlock + threadIdx.x;
+                    e[r] = j < nb ? r1[t0 + j] : 0xFFFFFFFFu;
+                }
+#pragma unroll
+                for (uint32_t r = 0; r < FU; ++r) {
+                    fr[r] = 0xFFFFFFFFu;
+                    if (e[r] >= w0 && e[r] < w1) {
+                        const uint32_t f = (e[r] - w0) / vbw;
+                        fr[r] = (f << 16) | atomicAdd(&sh.fhist[f], 1u);
+                    }
+                }
+                __syncthreads();
+                uint32_t tot;
+                {
+                    const uint32_t x = threadIdx.x < nf ? sh.fhist[threadIdx.x] : 0u;
+                    const uint32_t ex = block_excl_scan(x, &tot, sh.tmp);
+                    if (threadIdx.x < nf) {
+                        sh.fhist[threadIdx.x] = ex;
+                        sh.fwo[threadIdx.x] = sh.fcur[threadIdx.x] - ex;
+                        sh.fcur[threadIdx.x] += x;
+                    }
+                }
+                __syncthreads();
+#pragma unroll
+                for (uint32_t r = 0; r < FU; ++r)
+                    if (fr[r] != 0xFFFFFFFFu) sh.buf[sh.fhist[fr[r] >> 16] + (fr[r] & 0xFFFFu)] = e[r];
+                __syncthreads();
+                if (threadIdx.x < nf) sh.fhist[threadIdx.x] = 0;
+                for (uint32_t j = threadIdx.x; j < tot; j += kBlock) {
+                    const uint32_t x = sh.buf[j];
+                    r2[sh.fwo[(x - w0) / vbw] + j] = x;
+                }
+                __syncthreads();  // `buf` and `fwo` are refilled by the next batch
+            }
+            for (uint32_t f = threadIdx.x; f < nf; f += kBlock) fse[f] = make_uint2(sh.fstart[f], sh.fstart[f + 1]);
+        }
+        if (threadIdx.x == 0) {
+            bg.bh[q] = 0;
+            bg.bc[q] = 0;
+            if (nrec) atomicAdd(&states[i].brec[0], (unsigned long long)nrec);
+        }
+        __syncthreads();
+    }
+}
+
+// The fine bins of the binned instances, dealt interleaved over the grid
+// (block b takes fine bins b, b + G, ... of each instance, so every block
+// gets a share of the dense and the sparse ranges): each fine bin with
+// records is applied in LDS (bin_apply_fine), its removals pushed as chunks
+// of the round's list.
+template <uint32_t DW>
+__global__ void __launch_bounds__(kBlock) k_bin_fine(const Inst* insts, InstState* states, Control* ctl, uint32_t I,
+                                                     uint32_t s, uint32_t round, BinGeo bg) {
+    __shared__ BinApplyShared<DW> sh;
+    const uint32_t par = s & 1u, rp = round & 1u;
+    if (threadIdx.x < I) {
+        sh.mode[threadIdx.x] = states[threadIdx.x].bflag[rp];
+        sh.level[threadIdx.x] = states[threadIdx.x].dk[par];
+    }
+    for (uint32_t t = threadIdx.x; t < DW; t += kBlock) sh.delta[t] = 0;
+    for (uint32_t t = threadIdx.x; t < kBinMaxVb / 32; t += kBlock) sh.claim[t] = 0;
+    if (threadIdx.x == 0) {
         sh.removed = 0;
         sh.ent = 0;
         sh.nrem = 0;
     }
     __syncthreads();
-    const uint32_t ntasks = sh.first[I];
-    for (uint32_t t = blockIdx.x; t < ntasks; t += gridDim.x) {
-        uint32_t i = 0;
-        while (!(sh.mode[i] && t < sh.first[i] + bg.C)) ++i;  // t < ntasks: some binned instance holds it
-        const uint32_t c = t - sh.first[i];
-        bin_apply_task(insts[i], &states[i], ctl, bg, &sh, i, c, par, rp, sh.level[i]);
+    const uint32_t G = gridDim.x, vbw = bg.vbw;
+    const uint64_t rcap = bg.rec2 - bg.rec1;
+    for (uint32_t i = 0; i < I; ++i) {
+        if (!sh.mode[i]) continue;
+        const Inst& in = insts[i];
+        InstState* st = &states[i];
+        const int32_t k = sh.level[i];
+        const uint2* fse = bg.fse + uint64_t(i) * bg.nfine;
+        for (uint64_t j0 = 0; j0 * G + blockIdx.x < bg.nfine; j0 += kBinFineBatch) {
+            if (threadIdx.x == 0) sh.nlist = 0;
+            __syncthreads();
+            const uint64_t F = (j0 + threadIdx.x) * G + blockIdx.x;
+            if (threadIdx.x < kBinFineBatch && F < bg.nfine) {
+                const uint2 ab = fse[F];
+                if (ab.y > ab.x) {
+                    const uint32_t c = bg.fcoarse[F];
+                    const uint32_t w0 = bg.cbw[c], w1 = bg.cbw[c + 1];
+                    const uint32_t fw0 = w0 + (uint32_t(F) - bg.ffirst[c]) * vbw;
+                    const uint32_t at = atomicAdd(&sh.nlist, 1u);
+                    sh.fdesc[at] = make_uint4(ab.x, ab.y, fw0, w1 - fw0 < vbw ? w1 - fw0 : vbw);
+                    sh.foff[at] = (w1 - w0 <= vbw ? 0ull : rcap) + bg.bb[uint64_t(i) * bg.cmax + c];
+                }
+            }
+            __syncthreads();
+            const uint32_t nl = sh.nlist;
+            for (uint32_t m = 0; m < nl; ++m) {
+                const uint4 d = sh.fdesc[m];
+                bin_apply_fine(in, st, ctl, &sh, bg.rec1 + sh.foff[m], false, d.x, d.y, d.z, d.w, par, rp, k,
+                               bg.claim_q);
+            }
+            __syncthreads();  // the list is refilled next
+        }
+        bin_push_removed(in, st, ctl, &sh, par, rp);
         if (threadIdx.x == 0) {
-            if (sh.removed) atomicAdd(&states[i].alive[0], 0ull - (unsigned long long)sh.removed);
-            if (sh.ent) atomicAdd(&states[i].pber[rp][0], sh.ent);
+            if (sh.removed) atomicAdd(&st->alive[0], 0ull - (unsigned long long)sh.removed);
+            if (sh.ent) atomicAdd(&st->pber[rp][0], sh.ent);
             sh.removed = 0;
             sh.ent = 0;
         }

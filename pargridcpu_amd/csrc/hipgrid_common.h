@@ -209,7 +209,8 @@ inline uint64_t bin_rcap(uint64_t D) { return std::min<uint64_t>(3ull << 29, 2 *
 inline uint64_t bin_bytes(uint64_t D, uint64_t L) {
     if (!bin_enabled(D)) return 0;
     return pool_block_bytes(8 * bin_rcap(D)) + 2 * pool_block_bytes(4 * 1024 * L) + pool_block_bytes(8 * 1024 * L) +
-           pool_block_bytes(4 * 1025) + pool_block_bytes(2 * 4097);
+           pool_block_bytes(4 * 1025) + pool_block_bytes(2 * 4097) + pool_block_bytes(4 * 1025) +
+           pool_block_bytes(2ull * 1024 * 512) + pool_block_bytes(8ull * 1024 * 512 * L);
 }
 
 // Core-index bounds per entry (adjh): for each lmd, one byte per entry holding
