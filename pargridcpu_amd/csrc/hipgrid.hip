@@ -287,15 +287,15 @@ int bin_setup(hg_graph* g, Scratch& tmp, hgc::Staging& up) {
     bg.fse = g->bin_fse;
     bg.nfine = nfine;
     // binned when the frontier's entries reach HG_BIN_DENS / 1000 of the
-    // alive vertices' degree words (default 100, i.e. 10 %: on the
-    // 100M-vertex workload 496 ms per decomposition against 499 at 3 %)
+    // alive vertices' degree words, in up to HG_BIN_ROUNDS removal rounds
+    // per level (later rounds are sparser).  On the 100M-vertex workload
+    // 5 % and three rounds: 380-381 ms per decomposition; 10 % and two
+    // rounds 386 ms; 2.5 % and three rounds 395 ms; four rounds 380 ms.
     const char* dv = std::getenv("HG_BIN_DENS");
-    bg.dens_num = dv && *dv ? uint32_t(std::max(0, std::atoi(dv))) : 100u;
+    bg.dens_num = dv && *dv ? uint32_t(std::max(0, std::atoi(dv))) : 50u;
     bg.dens_den = 1000;
     g->bin_steps = uint32_t(env_int("HG_BIN_STEPS", 64));
-    // removal rounds per level that may bin: later rounds are sparser; on the
-    // 100M-vertex workload 1: 469.6 ms, 2: 466.6, 3: 466.1
-    g->bin_rounds = uint32_t(env_int("HG_BIN_ROUNDS", 2));
+    g->bin_rounds = uint32_t(env_int("HG_BIN_ROUNDS", 3));
     g->bin_eu = env_int("HG_BIN_EU", 4) == 8 ? 8u : 4u;
     g->bin_level0 = env_int("HG_BIN_L0", 1) == 1;
     bg.cont = env_int("HG_BIN_CONT", 1) == 1 ? 1u : 0u;
