@@ -429,7 +429,7 @@ def run_decompose(args, spec) -> dict:
             "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic, "traffic_source": traffic_src,
             "counted_hbm_gbs": round(traffic / (dev_ms * 1e-3) / 1e9, 1) if traffic else None,
             "l2_hit_rate": round(l2_hits / (l2_hits + l2_miss), 4) if l2_hits is not None and l2_miss else None,
-            "kernel": "hg_decompose (k_init + per-level k_scan/k_cascade + binned first rounds k_bin_expand/scatter/apply)", "algorithmic_bytes": balg,
+            "kernel": "hg_decompose (k_init + per-level k_scan/k_cascade + binned dense rounds k_bin_expand/scatter/group/fine)", "algorithmic_bytes": balg,
             "avg_ms": round(dev_ms, 4)}
     eng.close()
     del eng, g

@@ -134,7 +134,7 @@ struct hg_graph {
     uint32_t bin_eu = 4;     // entries per thread in flight in k_bin_expand (4 or 8)
     uint32_t bin_xg = 4;     // k_bin_expand blocks per CU
     bool bin_level0 = true;  // step 0 (level 0's pushed frontier) may bin too
-    uint32_t bin_sg = 2, bin_ag = 3;  // k_bin_scatter / k_bin_apply blocks per CU
+    uint32_t bin_sg = 2, bin_ag = 3;  // k_bin_scatter / k_bin_group and k_bin_fine blocks per CU
     BinGeo bgeo{};
     uint32_t* bin_cbw = nullptr;
     uint32_t* bin_bh = nullptr;
