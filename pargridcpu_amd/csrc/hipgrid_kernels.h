@@ -730,12 +730,14 @@ k_init(const Inst* insts, InstState* states, Control* ctl, uint32_t I,
     __shared__ ScanShared sh[64];
     __shared__ StepShared ss;
     // Pulled rows of light vertices are staged here (a byte suffices: a light
-    // vertex has at most kPullHeavy entries) and written out as whole runs of
-    // 64 rows, so every line of deg is written once, in full.  32 KB of
+    // vertex has at most kPullHeavy entries) and written out in runs of 64
+    // rows (the rows an instance never reads left out), so every line of deg
+    // is written once.  32 KB of
     // staging leaves room for three blocks per CU, the grid of every level
     // kernel (all of them resident at once).
     __shared__ uint8_t s_stage[kStageWords];   // [instance][candidate][layer]
     __shared__ uint8_t s_c1[NJ * kBlock];  // layers with a pulled count >= 1
+    __shared__ uint8_t s_zc[kBlock];       // z of each candidate (0 for none)
     const bool stage = pull && I * L * uint32_t(kBlock) <= kStageWords;
     for (uint32_t i = threadIdx.x; i < I; i += blockDim.x) {
         sh[i].keep = 0;
@@ -790,11 +792,13 @@ k_init(const Inst* insts, InstState* states, Control* ctl, uint32_t I,
                 }
             }
         }
-        if (stage && !(valid && zv > 0 && v >= H))  // rows without pulled counts are written as zeros
-            for (uint32_t t = 0; t < I * L; ++t) s_stage[((t / L) * kBlock + threadIdx.x) * L + t % L] = 0;
         if (stage) {
             // the wavefront's 64 candidates are one run of consecutive ids:
-            // its rows are 64 * L consecutive words of each instance's deg
+            // its rows are 64 * L consecutive words of each instance's deg.
+            // Instance j writes only the rows of vertices with z > j: no
+            // other vertex is ever alive there, and no kernel reads the row
+            // of a vertex that is not alive (hub rows come from k_pull_heavy)
+            s_zc[threadIdx.x] = uint8_t(valid ? zv : 0u);
             __syncthreads();
             const uint32_t w0 = threadIdx.x & ~uint32_t(kWave - 1);  // first candidate of this wavefront
             const uint64_t run0 = (uint64_t((base + w0) / kRun) * G + blockIdx.x) * kRun;
@@ -803,7 +807,8 @@ k_init(const Inst* insts, InstState* states, Control* ctl, uint32_t I,
                     uint32_t* rows = insts[j].deg + run0 * L;
                     for (uint32_t t = lane_id(); t < kRun * L; t += kWave) {
                         const uint64_t vt = run0 + t / L;
-                        if (vt < n && vt >= H) rows[t] = s_stage[(j * kBlock + w0 + t / L) * L + t % L];
+                        if (vt < n && vt >= H && s_zc[w0 + t / L] > j)
+                            rows[t] = s_stage[(j * kBlock + w0 + t / L) * L + t % L];
                     }
                 }
             }
