@@ -1,8 +1,9 @@
-// hipgrid_bin_kernels.h -- binned apply of a level's first removal round.
+// hipgrid_bin_kernels.h -- binned apply of a level's dense removal rounds.
 //
-// The first round of a level (the vertices its scan removes, F0) carries
-// most of a decomposition's decrements: on the 100M-vertex workload a level's
-// F0 decrements 10-35 % of the alive vertices' degree words at once.  Applied
+// The first rounds of a level (the vertices its scan removes, F0, and those
+// F0 removes, F1, ...) carry most of a decomposition's decrements: on the
+// 100M-vertex workload a level's F0 decrements 10-35 % of the alive
+// vertices' degree words at once.  Applied
 // as scattered device-scope atomics (one 128-byte line request each, ~18 G/s
 // on a 3.2 GB array whether or not the old value is returned) they are bound
 // by the atomic request rate.  Here they become records (the target word
