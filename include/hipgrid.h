@@ -168,7 +168,7 @@ typedef struct hg_timing {
     double densest_ms;    /* edge histogram + cell scoring of hg_densest */
     int64_t levels;       /* level iterations run */
     int64_t launches;     /* kernel launches in hg_decompose */
-    int64_t binned;       /* (instance, level) first rounds applied binned by target range in LDS */
+    int64_t binned;       /* (instance, round) pairs applied binned by target range in LDS */
 } hg_timing;
 int hg_last_timing(hg_graph* g, hg_timing* out);
 

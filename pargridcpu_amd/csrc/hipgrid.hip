@@ -126,10 +126,10 @@ struct hg_graph {
     int64_t cmp_bytes = 0;
     std::vector<Inst> cur;     // the instances' current rows during hg_decompose
     int64_t compactions = 0;   // in the last hg_decompose
-    // binned apply of dense first rounds (k_bin_*): coarse bin table, per-bin
+    // binned apply of dense removal rounds (k_bin_*): coarse bin table, per-bin
     // counts / cursors / region starts per instance, two record buffers
     bool bin_on = false;
-    uint32_t bin_steps = 0;  // steps 1..bin_steps may bin
+    uint32_t bin_steps = 0;  // steps 1..bin_steps may bin (and step 0 with bin_level0)
     uint32_t bin_rounds = 0; // removal rounds 0..bin_rounds-1 of such a step may bin
     uint32_t bin_eu = 4;     // entries per thread in flight in k_bin_expand (4 or 8)
     uint32_t bin_xg = 4;     // k_bin_expand blocks per CU
@@ -656,7 +656,7 @@ int launch_step(hg_graph* g, uint32_t s) {
         HIP_TRY(hipMemcpy(before.data(), g->d_states, sizeof(InstState) * L, hipMemcpyDeviceToHost));
     }
     if (g->bin_on && s <= g->bin_steps && (s >= 1 || g->bin_level0)) {
-        // dense first rounds: binned apply (the kernels return at once for
+        // dense removal rounds: binned apply (the kernels return at once for
         // instances that do not bin this step)
         const uint32_t g2 = 2u * uint32_t(g->cus);
         for (uint32_t r = 0; r < g->bin_rounds; ++r) {  // removal rounds 0, 1, ... of the level
@@ -719,8 +719,8 @@ int launch_step(hg_graph* g, uint32_t s) {
         }
         Control c{};
         HIP_TRY(hipMemcpy(&c, g->d_ctl, sizeof(Control), hipMemcpyDeviceToHost));
-        std::fprintf(stderr, "[hg] step %u count %.3f scatter %.3f apply %.3f cascade %.3f dbg %llu %llu %llu%s\n", s,
-                     tk[0], tk[1], tk[2], tk[3], c.dbg[0], c.dbg[1], c.dbg[2], line.c_str());
+        std::fprintf(stderr, "[hg] step %u expand %.3f scatter %.3f apply %.3f cascade %.3f scatter-batches %llu%s\n", s,
+                     tk[0], tk[1], tk[2], tk[3], c.dbg[0], line.c_str());
     }
     return HG_OK;
 }

@@ -139,7 +139,7 @@ def test_c3_full_size(capsys):
         f"{time.perf_counter() - t0:.0f} s")
     ci, cl, kmax, t = decompose(mg)
     say(f"  engine: {t.decompose_ms:.1f} ms, {t.binned} binned rounds, kmax={kmax.tolist()}")
-    assert t.binned > 0  # the default path bins the dense first rounds of this graph
+    assert t.binned > 0  # the default path bins the dense removal rounds of this graph
     check_structure(ci, cl, kmax)
     say(f"  structure ok at {time.perf_counter() - t0:.0f} s; oracle (one lmd per host thread)")
     check_oracle(mg, ci, say)
