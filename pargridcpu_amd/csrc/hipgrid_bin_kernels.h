@@ -484,7 +484,7 @@ __device__ __forceinline__ void bin_push_removed(const Inst& in, InstState* st, 
 }
 
 // Vertex uu of the fine bin starting at vertex fv0: subtracts its record
-// counts from its row (the task owns the row), clears them, stores the new
+// counts from its row (its fine bin owns the row), clears them, stores the new
 // count of layers at or above thr and removes the vertex (index k) when
 // fewer than lmd are left; returns whether it did, with the vertex in *u.
 template <class SH>
@@ -564,12 +564,12 @@ __device__ __forceinline__ bool bin_touched8(const SH* sh, uint32_t uu) {
 
 // The records [a, b) of one fine bin (degree words [fw0, fw0 + nw)): counted
 // per word in LDS, then a sweep over the bin's vertices: a vertex with any
-// count subtracts them from its row (the task owns these rows), clears them,
+// count subtracts them from its row (the fine bin owns these rows), clears them,
 // and recounts its layers at or above the threshold; with fewer than lmd it
 // is removed.  The counters are zero again afterwards.  All threads call.
 template <class SH>
 __device__ __forceinline__ void bin_apply_fine(const Inst& in, InstState* st, Control* ctl, SH* sh,
-                                               const uint32_t* recf, bool agent, uint32_t a, uint32_t b,
+                                               const uint32_t* recf, uint32_t a, uint32_t b,
                                                uint32_t fw0, uint32_t nw, uint32_t par, uint32_t rp, int32_t k,
                                                uint32_t claim_q) {
     const uint32_t L = in.L, thr = uint32_t(k) + 1u;
@@ -586,7 +586,7 @@ __device__ __forceinline__ void bin_apply_fine(const Inst& in, InstState* st, Co
 #pragma unroll
         for (uint32_t r = 0; r < kBinUnroll; ++r) {
             const uint32_t t = a + r * kBlock + threadIdx.x;
-            e[r] = t < b ? (agent ? ld_agent(&recf[t]) : recf[t]) : 0xFFFFFFFFu;
+            e[r] = t < b ? recf[t] : 0xFFFFFFFFu;
         }
 #pragma unroll
         for (uint32_t r = 0; r < kBinUnroll; ++r)
@@ -613,7 +613,7 @@ __device__ __forceinline__ void bin_apply_fine(const Inst& in, InstState* st, Co
 #pragma unroll
         for (uint32_t r = 0; r < kBinUnroll; ++r) {
             const uint32_t t = t0 + r * kBlock;
-            e[r] = t < b ? (agent ? ld_agent(&recf[t]) : recf[t]) : 0xFFFFFFFFu;
+            e[r] = t < b ? recf[t] : 0xFFFFFFFFu;
         }
 #pragma unroll
         for (uint32_t r = 0; r < kBinUnroll; ++r)
@@ -627,7 +627,7 @@ __device__ __forceinline__ void bin_apply_fine(const Inst& in, InstState* st, Co
             const uint32_t t = t0 + threadIdx.x;
             bool rem = false;
             uint32_t u = 0;
-            const uint32_t e = t < b ? (agent ? ld_agent(&recf[t]) : recf[t]) : 0xFFFFFFFFu;
+            const uint32_t e = t < b ? recf[t] : 0xFFFFFFFFu;
             if (e - fw0 < nw) {
                 const uint32_t uu = div_L(e - fw0, L, in.lmagic);
                 const uint32_t bit = 1u << (uu & 31u);
@@ -875,7 +875,7 @@ __global__ void __launch_bounds__(kBlock) k_bin_fine(const Inst* insts, InstStat
             const uint32_t nl = sh.nlist;
             for (uint32_t m = 0; m < nl; ++m) {
                 const uint4 d = sh.fdesc[m];
-                bin_apply_fine(in, st, ctl, &sh, bg.rec1 + sh.foff[m], false, d.x, d.y, d.z, d.w, par, rp, k,
+                bin_apply_fine(in, st, ctl, &sh, bg.rec1 + sh.foff[m], d.x, d.y, d.z, d.w, par, rp, k,
                                bg.claim_q);
             }
             __syncthreads();  // the list is refilled next
